@@ -1,0 +1,1451 @@
+// engine.hip — HIP kernels (gfx950, wave64) and the host engine of the
+// concurrent multi-query BFS / SSSP / WCC traversal.
+//
+// Reference behaviour followed (paths relative to the reference root):
+//   push advance with per-vertex query masks
+//     puercgp/include/puercgp/engine/frontier_engine.hxx:678-943 (shared-node push),
+//     hybrid_engine.hxx:132-330 (mixed BFS/SSSP bits in one traversal);
+//   degree-binned mapping  frontier_engine.hxx:945-1235;
+//   next-frontier marking  frontier_engine.hxx:117-173;
+//   relaxation semantics   core/reduce_ops.hxx:24-110 (BFS first-write with the
+//                          query-relative depth source_value+1, SSSP min-reduce);
+//   pull (bottom-up) BFS   frontier_engine.hxx:1415-1486, thresholds core/types.hxx:30-31;
+//   host loop              frontier_engine.hxx:2210-3139, hybrid_engine.hxx:796-1140.
+// WCC uses hooking plus pointer jumping instead of the reference's label
+// propagation (algorithms/wcc.hxx:20-42); both yield the minimum vertex id of
+// each component on symmetric graphs.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cgp_internal.hpp"
+#include "kernels.hpp"
+
+using namespace cgpk;
+
+#define HIP_CHECK(x)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      cgpi::fail(e_ == hipErrorOutOfMemory ? CGP_ENOMEM : CGP_EDEVICE,                \
+                 std::string(#x) + ": " + hipGetErrorString(e_));                     \
+  } while (0)
+
+// ============================================================== device helpers
+namespace {
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Number of set bits of m below the calling lane.
+__device__ __forceinline__ u32 rank_below(u64 m) {
+  return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
+__device__ __forceinline__ u64 shfl64(u64 x, int src) {
+  u32 lo = (u32)__shfl((int)(u32)x, src, 64);
+  u32 hi = (u32)__shfl((int)(u32)(x >> 32), src, 64);
+  return ((u64)hi << 32) | lo;
+}
+
+// Wave-aggregated append of one item per predicated lane: one atomicAdd per
+// wave, slots handed out by the lanes' rank among the predicated lanes.
+// Safe under divergence (the ballot sees the active lanes only).
+template <typename T>
+__device__ __forceinline__ void wave_append(bool pred, T item, u32* counter, T* list) {
+  u64 m = __ballot(pred);
+  if (m == 0) return;
+  int leader = __ffsll((long long)m) - 1;
+  u32 base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (u32)__popcll(m));
+  base = (u32)__shfl((int)base, leader, 64);
+  if (pred) list[base + rank_below(m)] = item;
+}
+
+// Wave-level exclusive prefix sum of per-lane counts, with one atomicAdd per
+// wave reserving the total; returns the lane's first slot.  Every lane of the
+// wave must call it.
+__device__ __forceinline__ u32 wave_reserve(u32 cnt, u32* counter) {
+  u32 x = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    u32 y = (u32)__shfl_up((int)x, d, 64);
+    if (lane_id() >= d) x += y;
+  }
+  u32 total = (u32)__shfl((int)x, 63, 64);
+  u32 base = 0;
+  if (total != 0 && lane_id() == 0) base = atomicAdd(counter, total);
+  base = (u32)__shfl((int)base, 0, 64);
+  return base + x - cnt;
+}
+
+__device__ __forceinline__ u64 wave_sum64(u64 x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += shfl64(x, lane_id() ^ d);
+  return x;
+}
+
+__device__ __forceinline__ u64 wave_or64(u64 x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x |= shfl64(x, lane_id() ^ d);
+  return x;
+}
+
+// First claim of vertex v in the iteration whose stamp is `stamp`.
+__device__ __forceinline__ bool claim_vertex(u32* mark, int32_t v, u32 stamp) {
+  if (__hip_atomic_load(&mark[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == stamp)
+    return false;
+  return atomicExch(&mark[v], stamp) != stamp;
+}
+
+__device__ __forceinline__ size_t vrow(int32_t V, int w, int32_t v, int rshift) {
+  return ((size_t)w * (size_t)V + (size_t)v) << rshift;
+}
+
+__device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
+  x = wave_sum64(x);
+  if (lane_id() == 0 && x) atomicAdd(&P.ctl->prof[slot], x);
+}
+
+// ------------------------------------------------------------------ push
+// Relaxes edge (u -> v, weight wv) for every active BFS bit (bitmask
+// first-write against visited) and every sparse SSSP bit (atomicMin on the
+// value); sets the improved bits in next[v] and reports whether this lane
+// made the vertex's first claim of the iteration.
+template <int W>
+__device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u32 wv,
+                                           const u64 (&bm)[W], const u64 (&sm)[W], u32 stamp,
+                                           int rshift, u64& nbfs, u64& nsssp, u64& nnew,
+                                           u64& nimp) {
+  bool first = false;
+  const size_t vb = (size_t)v * W;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    u64 add = 0;
+    if (bm[w]) {
+      ++nbfs;
+      add = bm[w] & ~P.visited[vb + w];
+    }
+    if (sm[w]) {
+      u64 s = sm[w];
+      const size_t ub = vrow(P.V, w, u, rshift), vr = vrow(P.V, w, v, rshift);
+      while (s) {
+        const int q = __ffsll((long long)s) - 1;
+        s &= s - 1;
+        ++nsssp;
+        const u32 du = P.vals[ub + q];
+        const u32 cand = du + wv;
+        if (cand < du) continue;  // saturate: never wrap past CGP_INF
+        u32* pv = &P.vals[vr + q];
+        if (cand < *pv) {
+          const u32 old = atomicMin(pv, cand);
+          if (cand < old) {
+            add |= 1ull << q;
+            ++nimp;
+          }
+        }
+      }
+    }
+    if (add) {
+      u64* pn = &P.next[vb + w];
+      if (add & ~*pn) {
+        const u64 old = atomicOr(pn, add);
+        if (add & ~old) ++nnew;
+        if (old == 0) first = true;
+      }
+    }
+  }
+  return first && claim_vertex(P.mark, v, stamp);
+}
+
+template <int W>
+__device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, int dense_t,
+                                           u64 (&bm)[W], u64 (&sm)[W], bool& any_w) {
+  const Cfg* cfg = P.cfg;
+  any_w = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const u64 f = P.front[(size_t)u * W + w];
+    bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
+    u64 s = f & cfg->val_bits[w];
+    if (__popcll(s) >= dense_t) s = 0;  // handled by the lane-per-query kernel
+    sm[w] = s;
+    any_w |= (s != 0);
+  }
+}
+
+// G lanes per frontier vertex (G = 8 for the small bin, 64 for the medium bin).
+template <int W, int G>
+__global__ __launch_bounds__(BLOCK) void k_push_group(KP P, int which) {
+  Ctl* C = P.ctl;
+  if (C->finished) return;
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 n = C->bcount[p][which];
+  if (n == 0) return;
+  const int32_t* list = which == 0 ? P.small[p] : P.medium[p];
+  const bool bfs_on = C->bfs_pull == 0;
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
+  const int gl = (int)(threadIdx.x % G);
+  const u32 gid = (blockIdx.x * BLOCK + threadIdx.x) / G;
+  const u32 ngroups = (gridDim.x * BLOCK) / G;
+  u64 nitems = 0, nedges = 0, nwedges = 0, nbfs = 0, nsssp = 0, nnew = 0, nimp = 0;
+  for (u32 i = gid; i < n; i += ngroups) {
+    const int32_t u = list[i];
+    u64 bm[W], sm[W];
+    bool any_s;
+    load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+    bool any_b = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
+    if (!any_b && !any_s) continue;
+    if (gl == 0) ++nitems;
+    const int32_t e0 = P.off[u], e1 = P.off[u + 1];
+    for (int32_t e = e0 + gl; e < e1; e += G) {
+      const int32_t v = P.idx[e];
+      u32 wv = 1;
+      if (any_s && P.wt) {
+        wv = P.wt[e];
+        ++nwedges;
+      }
+      ++nedges;
+      const bool claimed = relax_edge<W>(P, u, v, wv, bm, sm, stamp, rshift, nbfs, nsssp, nnew, nimp);
+      wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+    }
+  }
+  if (P.profile) {
+    prof_add(P, PF_PUSH_ITEMS, nitems);
+    prof_add(P, PF_PUSH_EDGES, nedges);
+    prof_add(P, PF_PUSH_WEDGES, nwedges);
+    prof_add(P, PF_BFS_WORDS, nbfs);
+    prof_add(P, PF_SSSP_PAIRS, nsssp);
+    prof_add(P, PF_NEW_WORDS, nnew);
+    prof_add(P, PF_IMPROVED, nimp);
+  }
+}
+
+// One workgroup per LCHUNK-edge chunk of a high-degree frontier vertex.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_push_large(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished) return;
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 n = C->bcount[p][2];
+  if (n == 0) return;
+  const bool bfs_on = C->bfs_pull == 0;
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
+  u64 nitems = 0, nedges = 0, nwedges = 0, nbfs = 0, nsssp = 0, nnew = 0, nimp = 0;
+  for (u32 i = blockIdx.x; i < n; i += gridDim.x) {
+    const u64 item = P.large[p][i];
+    const int32_t u = (int32_t)(item >> 32);
+    const u32 chunk = (u32)item;
+    u64 bm[W], sm[W];
+    bool any_s;
+    load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+    bool any_b = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
+    if (!any_b && !any_s) continue;
+    if (threadIdx.x == 0) ++nitems;
+    const int32_t e0 = P.off[u] + (int32_t)(chunk * LCHUNK);
+    const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
+    for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += BLOCK) {
+      const int32_t v = P.idx[e];
+      u32 wv = 1;
+      if (any_s && P.wt) {
+        wv = P.wt[e];
+        ++nwedges;
+      }
+      ++nedges;
+      const bool claimed = relax_edge<W>(P, u, v, wv, bm, sm, stamp, rshift, nbfs, nsssp, nnew, nimp);
+      wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+    }
+  }
+  if (P.profile) {
+    prof_add(P, PF_PUSH_ITEMS, nitems);
+    prof_add(P, PF_PUSH_EDGES, nedges);
+    prof_add(P, PF_PUSH_WEDGES, nwedges);
+    prof_add(P, PF_BFS_WORDS, nbfs);
+    prof_add(P, PF_SSSP_PAIRS, nsssp);
+    prof_add(P, PF_NEW_WORDS, nnew);
+    prof_add(P, PF_IMPROVED, nimp);
+  }
+}
+
+// Dense SSSP words: one wave per (vertex, word, DCHUNK-edge chunk), lane q is
+// query q of the word, the wave walks the edges in order and each lane's
+// value access is one coalesced 256 B row per edge.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_push_dense(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished) return;
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 n = C->bcount[p][3];
+  if (n == 0) return;
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift;
+  const int R = 1 << rshift;
+  const int lane = lane_id();
+  const u32 wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const u32 nwaves = (gridDim.x * BLOCK) >> 6;
+  u64 nitems = 0, nedges = 0, nsssp = 0, nnew = 0, nimp = 0;
+  for (u32 i = wave; i < n; i += nwaves) {
+    const u64 item = P.dense[p][i];
+    const int32_t u = (int32_t)(item >> 32);
+    const int w = (int)((item >> 24) & 0xFF);
+    const u32 chunk = (u32)(item & 0xFFFFFF);
+    const u64 sm = P.front[(size_t)u * W + w] & P.cfg->val_bits[w];
+    const int32_t e0 = P.off[u] + (int32_t)(chunk * DCHUNK);
+    const int32_t e1 = min(P.off[u + 1], e0 + DCHUNK);
+    const bool act = lane < R && ((sm >> lane) & 1ull);
+    const u32 du = act ? P.vals[vrow(P.V, w, u, rshift) + lane] : INF;
+    if (lane == 0) {
+      ++nitems;
+      nedges += (u64)(e1 - e0);
+      nsssp += (u64)(e1 - e0) * (u64)__popcll(sm);
+    }
+    for (int32_t eb = e0; eb < e1; eb += 64) {
+      const int cnt = min(64, e1 - eb);
+      const int32_t myv = lane < cnt ? P.idx[eb + lane] : 0;
+      const u32 myw = lane < cnt ? (P.wt ? P.wt[eb + lane] : 1u) : 0u;
+      int j = 0;
+      for (; j + 4 <= cnt; j += 4) {
+        int32_t vv[4];
+        u32 cand[4], cur[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          vv[k] = __builtin_amdgcn_readlane(myv, j + k);
+          const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, j + k);
+          cand[k] = du + wk;
+          if (cand[k] < du) cand[k] = INF;
+          cur[k] = act ? P.vals[vrow(P.V, w, vv[k], rshift) + lane] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          bool imp = false;
+          if (act && cand[k] < cur[k]) {
+            const u32 old = atomicMin(&P.vals[vrow(P.V, w, vv[k], rshift) + lane], cand[k]);
+            imp = cand[k] < old;
+          }
+          const u64 m = __ballot(imp);
+          if (m) {
+            nimp += imp ? 1 : 0;
+            bool claimed = false;
+            if (lane == 0) {
+              const u64 old = atomicOr(&P.next[(size_t)vv[k] * W + w], m);
+              if (m & ~old) ++nnew;
+              if (old == 0) claimed = claim_vertex(P.mark, vv[k], stamp);
+            }
+            wave_append<int32_t>(claimed, vv[k], &C->tcount, P.touched);
+          }
+        }
+      }
+      for (; j < cnt; ++j) {
+        const int32_t v = __builtin_amdgcn_readlane(myv, j);
+        const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, j);
+        u32 cand = du + wk;
+        if (cand < du) cand = INF;
+        bool imp = false;
+        if (act) {
+          u32* pv = &P.vals[vrow(P.V, w, v, rshift) + lane];
+          if (cand < *pv) {
+            const u32 old = atomicMin(pv, cand);
+            imp = cand < old;
+          }
+        }
+        const u64 m = __ballot(imp);
+        if (m) {
+          nimp += imp ? 1 : 0;
+          bool claimed = false;
+          if (lane == 0) {
+            const u64 old = atomicOr(&P.next[(size_t)v * W + w], m);
+            if (m & ~old) ++nnew;
+            if (old == 0) claimed = claim_vertex(P.mark, v, stamp);
+          }
+          wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+        }
+      }
+    }
+  }
+  if (P.profile) {
+    prof_add(P, PF_DENSE_ITEMS, nitems);
+    prof_add(P, PF_DENSE_EDGES, nedges);
+    prof_add(P, PF_SSSP_PAIRS, nsssp);
+    prof_add(P, PF_NEW_WORDS, nnew);
+    prof_add(P, PF_IMPROVED, nimp);
+  }
+}
+
+// ------------------------------------------------------------------ pull
+// Bottom-up BFS step on a symmetric graph: an unvisited (v, q) becomes
+// visited at depth L+1 iff some neighbour u has q visited (all visited bits
+// at this point have depth <= L, and a neighbour at depth < L would already
+// have visited v).  visited is not written until the filter runs, so the
+// scan reads a consistent level.  One lane per vertex with early exit once
+// every missing bit was found.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_pull(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished || C->bfs_pull == 0) return;
+  const u32 iter = C->iter;
+  const u32 stamp = C->stamp_base + iter + 1;
+  u64 bb[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
+  u64 nverts = 0, nwords = 0, nedges = 0, nnew = 0;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < P.V; base += (int64_t)gridDim.x * BLOCK) {
+    const int32_t v = (int32_t)(base + threadIdx.x);
+    const bool valid = v < P.V;
+    u64 tgt[W], acc[W];
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      tgt[w] = valid ? (bb[w] & ~P.visited[(size_t)v * W + w]) : 0ull;
+      acc[w] = 0;
+      any |= tgt[w] != 0;
+    }
+    bool claimed = false;
+    if (any) {
+      ++nverts;
+      const int32_t e1 = P.off[v + 1];
+      for (int32_t e = P.off[v]; e < e1; ++e) {
+        const int32_t u = P.idx[e];
+        ++nedges;
+        bool done = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          if (tgt[w] & ~acc[w]) {
+            ++nwords;
+            acc[w] |= P.visited[(size_t)u * W + w] & tgt[w];
+            done &= (acc[w] == tgt[w]);
+          }
+        }
+        if (done) break;
+      }
+      bool first = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        if (acc[w]) {
+          const u64 old = atomicOr(&P.next[(size_t)v * W + w], acc[w]);
+          ++nnew;
+          if (old == 0) first = true;
+        }
+      }
+      claimed = first && claim_vertex(P.mark, v, stamp);
+    }
+    wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+  }
+  if (P.profile) {
+    prof_add(P, PF_PULL_VERTS, nverts);
+    prof_add(P, PF_PULL_WORDS, nwords);
+    prof_add(P, PF_PULL_EDGES, nedges);
+    prof_add(P, PF_NEW_WORDS, nnew);
+  }
+}
+
+// ------------------------------------------------------------------ seed
+// Sets the source bit of every slot in next[] (and value 0 for SSSP slots)
+// and appends the sources to the touched list; the filter then turns them
+// into iteration 0's frontier exactly as it does for any other iteration.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, int nslots) {
+  Ctl* C = P.ctl;
+  const int s = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  const u32 stamp = C->stamp_base;  // stamp of iteration 0xFFFFFFFF
+  bool claimed = false;
+  int32_t v = 0;
+  if (s < nslots && slot_src[s] >= 0) {
+    v = slot_src[s];
+    const int w = s >> 6, q = s & 63;
+    atomicOr(&P.next[(size_t)v * W + w], 1ull << q);
+    if ((P.cfg->val_bits[w] >> q) & 1ull) P.vals[vrow(P.V, w, v, P.cfg->rshift) + q] = 0u;
+    claimed = claim_vertex(P.mark, v, stamp);
+  }
+  wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+}
+
+// ------------------------------------------------------------------ filter
+// Claims the next words of every touched vertex, settles BFS bits (visited
+// |= new, depth = iteration+1 - slot start, written as coalesced rows),
+// publishes the frontier words, and emits the degree-binned next frontier
+// with wave-level prefix sums.  The last workgroup to finish advances the
+// iteration, resets the consumed counters, and picks push or pull.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished) return;
+  __shared__ int s_last;
+  const Cfg* cfg = P.cfg;
+  const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
+  const u32 n = C->tcount;
+  const int rshift = cfg->rshift, dense_t = cfg->dense_t;
+  const int lane = lane_id();
+  u64 bb[W], vb[W], un[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    bb[w] = cfg->bfs_bits[w];
+    vb[w] = cfg->val_bits[w];
+    un[w] = 0;
+  }
+  const u32 depth_base = iter + 1u;
+  u64 pairs = 0, epairs = 0;
+  for (u32 base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+    const u32 i = base + threadIdx.x;
+    const bool valid = i < n;
+    const int32_t v = valid ? P.touched[i] : 0;
+    const int32_t deg = valid ? P.off[v + 1] - P.off[v] : 0;
+    u64 nb[W];
+    bool need_bin = false;
+    u32 dmask = 0, vpairs = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      u64 nw = 0;
+      if (valid) {
+        u64* pn = &P.next[(size_t)v * W + w];
+        nw = *pn;
+        if (nw) *pn = 0;
+        P.front[(size_t)v * W + w] = nw;
+      }
+      nb[w] = nw & bb[w];
+      if (nb[w]) P.visited[(size_t)v * W + w] |= nb[w];
+      const u64 s = nw & vb[w];
+      if (s && __popcll(s) >= dense_t) dmask |= 1u << w;
+      else if (s) need_bin = true;
+      if (nb[w]) need_bin = true;
+      un[w] |= nw;
+      vpairs += (u32)__popcll(nb[w]);
+    }
+    pairs += vpairs;
+    epairs += (u64)vpairs * (u64)deg;
+    if (deg == 0) {
+      need_bin = false;
+      dmask = 0;
+    }
+    const int b = deg <= P.bin_small ? 0 : (deg <= P.bin_large ? 1 : 2);
+    wave_append<int32_t>(need_bin && b == 0, v, &C->bcount[np][0], P.small[np]);
+    wave_append<int32_t>(need_bin && b == 1, v, &C->bcount[np][1], P.medium[np]);
+    {
+      const u32 lch = (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
+      const u32 at = wave_reserve(lch, &C->bcount[np][2]);
+      for (u32 c = 0; c < lch; ++c) P.large[np][at + c] = ((u64)(u32)v << 32) | c;
+    }
+    {
+      const u32 dch = (u32)((deg + DCHUNK - 1) / DCHUNK);
+      const u32 cnt = (u32)__popcll(dmask) * dch;
+      const u32 at = wave_reserve(cnt, &C->bcount[np][3]);
+      u32 k = 0;
+      for (int w = 0; w < W; ++w) {
+        if (!((dmask >> w) & 1u)) continue;
+        for (u32 c = 0; c < dch; ++c)
+          P.dense[np][at + k++] = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
+      }
+    }
+    // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
+    {
+      const int per = 64 >> rshift;
+      const int sub = lane >> rshift;
+      const int q = lane & ((1 << rshift) - 1);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const u64 any = __ballot(nb[w] != 0);
+        if (!any) continue;
+        const u32 dval = depth_base - (u32)cfg->start[w * 64 + q];
+        for (int j0 = 0; j0 < 64; j0 += per) {
+          const u64 grp = per == 64 ? ~0ull : (((1ull << per) - 1) << j0);
+          if (!(any & grp)) continue;
+          const int j = j0 + sub;
+          const u64 nbj = shfl64(nb[w], j);
+          const int32_t vj = __shfl(v, j, 64);
+          if ((nbj >> q) & 1ull) P.vals[vrow(P.V, w, vj, rshift) + q] = dval;
+        }
+      }
+    }
+  }
+  // Block-level reductions of the frontier statistics.
+  pairs = wave_sum64(pairs);
+  epairs = wave_sum64(epairs);
+  if (lane == 0) {
+    if (pairs) atomicAdd(&C->bfs_pairs[np], pairs);
+    if (epairs) atomicAdd(&C->bfs_edges[np], epairs);
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const u64 x = wave_or64(un[w]);
+    if (lane == 0 && x) atomicOr(&C->union_[w], x);
+  }
+  // Last workgroup: advance the state machine.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const u32 t = atomicAdd(&C->done_blocks, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    const u32 total = atomicAdd(&C->bcount[np][0], 0u) + atomicAdd(&C->bcount[np][1], 0u) +
+                      atomicAdd(&C->bcount[np][2], 0u) + atomicAdd(&C->bcount[np][3], 0u);
+    const u64 fp = atomicAdd(&C->bfs_pairs[np], 0ull), fe = atomicAdd(&C->bfs_edges[np], 0ull);
+    for (int w = 0; w < MAXW; ++w) {
+      C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
+    }
+    C->tcount = 0;
+    C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = 0;
+    C->bfs_pairs[p] = 0;
+    C->bfs_edges[p] = 0;
+    C->done_blocks = 0;
+    u32 pull = 0;
+    if (fp > 0) {
+      if (cfg->pull_policy == 1) pull = 1;
+      else if (cfg->pull_policy == 2) {
+        if (C->bfs_pull) pull = fp >= cfg->push_pairs_thr ? 1u : 0u;
+        else pull = (fp >= cfg->pull_pairs_thr || fe >= cfg->pull_edges_thr) ? 1u : 0u;
+      }
+    }
+    C->bfs_pull = pull;
+    const u32 next_iter = iter + 1u;
+    C->iter = next_iter;
+    if (total == 0 || (C->max_iter != 0 && next_iter >= C->max_iter)) {
+      C->finished = 1;
+    } else {
+      if (pull) ++C->pull_iters;
+      else ++C->push_iters;
+    }
+    __threadfence();
+  }
+}
+
+// ------------------------------------------------------------------ utilities
+__global__ void k_fill_u32(u32* p, size_t n, u32 val) {
+  const size_t stride = (size_t)gridDim.x * BLOCK;
+  size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  // 16-byte stores over the aligned body.
+  uint4* p4 = reinterpret_cast<uint4*>(p);
+  const size_t n4 = n / 4;
+  const uint4 v4 = make_uint4(val, val, val, val);
+  for (size_t k = i; k < n4; k += stride) p4[k] = v4;
+  for (size_t k = n4 * 4 + i; k < n; k += stride) p[k] = val;
+}
+
+// Transposes the vertex-major value rows of word w into query-major output
+// rows: out[qmap[w*64+q] * V + v].  One workgroup per 64-vertex tile.
+__global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, int32_t V, int rshift,
+                                                  const int32_t* qmap, u32* out) {
+  __shared__ u32 tile[64][65];
+  const int w = blockIdx.y;
+  const int R = 1 << rshift;
+  const int32_t v0 = (int32_t)blockIdx.x * 64;
+  const int nv = min(64, V - v0);
+  const size_t rowbase = vrow(V, w, v0, rshift);
+  for (int t = threadIdx.x; t < 64 * R; t += BLOCK) {
+    const int j = t >> rshift, q = t & (R - 1);
+    if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 64 * R; t += BLOCK) {
+    const int q = t >> 6, j = t & 63;
+    const int32_t dst = qmap[w * 64 + q];
+    if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = tile[q][j];
+  }
+}
+
+// rows[e] = source vertex of CSR slot e (one wave per vertex row).
+__global__ __launch_bounds__(BLOCK) void k_fill_rows(const int32_t* off, int32_t V, int32_t* rows) {
+  const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * BLOCK) >> 6;
+  for (int64_t v = wave; v < V; v += nw)
+    for (int32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) rows[e] = (int32_t)v;
+}
+
+// --------------------------------------------------------------- WCC
+// Hooking plus pointer jumping over parent pointers with parent[x] <= x at all
+// times: a hook links the larger of two roots under the smaller with a
+// compare-and-swap, so the root of every tree is its smallest vertex and the
+// final labels are the minimum vertex id of each weakly connected component.
+__device__ __forceinline__ int32_t load_parent(int32_t* parent, int32_t x) {
+  return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving (each visited node is re-pointed to its grandparent).
+__device__ __forceinline__ int32_t find_root(int32_t* parent, int32_t x) {
+  int32_t p = load_parent(parent, x);
+  while (p != x) {
+    const int32_t gp = load_parent(parent, p);
+    if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+    p = gp;
+    if (p == x) break;
+    p = load_parent(parent, x);
+  }
+  return x;
+}
+
+__global__ void k_wcc_init(int32_t* parent, int32_t V) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK)
+    parent[v] = (int32_t)v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wcc_hook(const int32_t* rows, const int32_t* idx, int64_t E,
+                                                    int32_t* parent, int half) {
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * BLOCK) {
+    const int32_t a = rows[e], b = idx[e];
+    if (a == b || (half && a < b)) continue;  // symmetric: each undirected edge once
+    int32_t ra = find_root(parent, a), rb = find_root(parent, b);
+    while (ra != rb) {
+      const int32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+      const int32_t old = atomicCAS(&parent[hi], hi, lo);
+      if (old == hi) break;
+      // hi stopped being a root: continue from its current ancestors.
+      ra = find_root(parent, old);
+      rb = find_root(parent, lo);
+    }
+  }
+}
+
+__global__ void k_wcc_compress(int32_t* parent, int32_t V) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    int32_t r = (int32_t)v;
+    int32_t p = parent[r];
+    while (p != r) {
+      r = p;
+      p = parent[r];
+    }
+    parent[v] = r;
+  }
+}
+
+// Writes the component labels into every WCC output row.
+__global__ void k_wcc_rows(const int32_t* parent, int32_t V, const int32_t* rows_out, int nrows, u32* out) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    const u32 l = (u32)parent[v];
+    for (int r = 0; r < nrows; ++r) out[(size_t)rows_out[r] * V + v] = l;
+  }
+}
+
+// m_q per output row: sum of out-degrees over vertices with a finite value.
+__global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, int32_t V, const u32* out, u64* m) {
+  const int r = blockIdx.y;
+  u64 acc = 0;
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK)
+    if (out[(size_t)r * V + v] != INF) acc += (u64)(off[v + 1] - off[v]);
+  acc = wave_sum64(acc);
+  if (lane_id() == 0 && acc) atomicAdd(&m[r], acc);
+}
+
+}  // namespace
+
+// ============================================================== host side
+struct cgp_graph {
+  int32_t V = 0;
+  int64_t E = 0;
+  bool symmetric = false;
+  bool weighted = false;
+  int32_t* off = nullptr;
+  int32_t* idx = nullptr;
+  int32_t* rows = nullptr;
+  u32* wt = nullptr;
+};
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grows to at least `need` bytes; returns true when the address changed.
+  bool ensure(size_t need) {
+    if (need <= bytes) return false;
+    release();
+    if (need == 0) return false;
+    HIP_CHECK(hipMalloc(&p, need));
+    bytes = need;
+    return true;
+  }
+};
+
+int device_cus() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  int cus = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus > 0 ? cus : 1;
+}
+
+void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    cgpi::fail(CGP_EDEVICE, "no usable HIP device (the engine needs a gfx950 GPU)");
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+int round_words(int w) { return w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : 8; }
+
+}  // namespace
+
+struct cgp_engine {
+  const cgp_graph* g = nullptr;
+  cgp_engine_opts opts{};
+  hipStream_t s = nullptr;
+  int cus = 1;
+  int wcap = 0;            // words allocated in visited/next/front/dense
+  DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
+  DevBuf parent, wcc_rows, outbuf, qmap, slot_src, prof_out;
+  Ctl* d_ctl = nullptr;
+  Cfg* d_cfg = nullptr;
+  Ctl* h_ctl = nullptr;     // pinned
+  Cfg* h_cfg = nullptr;     // pinned
+  int32_t* h_stage = nullptr;  // pinned staging (slot sources, qmap, wcc rows)
+  size_t h_stage_cap = 0;
+  u32 stamp_epoch = 1;
+  bool next_dirty = false;
+  bool busy = false;
+  std::mutex mu;
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  int gexec_iters[4] = {0, 0, 0, 0};
+  // Per-launch timing (profile mode).
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  // Serving session state lives in serve.hip (attached here).
+  void* serve = nullptr;
+
+  KP kp() const {
+    KP P{};
+    P.V = g->V;
+    P.E = g->E;
+    P.profile = opts.profile;
+    P.off = g->off;
+    P.idx = g->idx;
+    P.wt = g->weighted ? g->wt : nullptr;
+    P.rows = g->rows;
+    P.visited = (u64*)visited.p;
+    P.next = (u64*)next.p;
+    P.front = (u64*)front.p;
+    P.mark = (u32*)mark.p;
+    P.vals = (u32*)vals.p;
+    P.touched = (int32_t*)touched.p;
+    for (int k = 0; k < 2; ++k) {
+      P.small[k] = (int32_t*)small[k].p;
+      P.medium[k] = (int32_t*)medium[k].p;
+      P.large[k] = (u64*)large[k].p;
+      P.dense[k] = (u64*)dense[k].p;
+    }
+    P.ctl = d_ctl;
+    P.cfg = d_cfg;
+    P.bin_small = opts.bin_small;
+    P.bin_large = opts.bin_large;
+    return P;
+  }
+
+  void drop_graphs() {
+    for (int k = 0; k < 4; ++k) {
+      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+      gexec[k] = nullptr;
+      gexec_iters[k] = 0;
+    }
+  }
+
+  // Grid sizes: persistent-style, sized to the device, independent of the
+  // frontier so the launches can be captured once and replayed.
+  int grid_group() const { return cus * 8; }
+  int grid_large() const { return cus * 4; }
+  int grid_dense() const { return cus * 8; }
+  int grid_pull() const {
+    int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 16));
+  }
+  int grid_filter() const {
+    int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
+  }
+
+  void ensure_words(int W) {
+    const size_t V = (size_t)g->V, E = (size_t)g->E;
+    bool moved = false;
+    if (W > wcap) {
+      moved |= visited.ensure(V * W * 8);
+      moved |= next.ensure(V * W * 8);
+      moved |= front.ensure(V * W * 8);
+      HIP_CHECK(hipMemsetAsync(next.p, 0, V * W * 8, s));
+      for (int k = 0; k < 2; ++k) moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
+      wcap = W;
+    }
+    if (!mark.p) {
+      moved |= mark.ensure(V * 4 + 4);
+      HIP_CHECK(hipMemsetAsync(mark.p, 0, V * 4 + 4, s));
+      moved |= touched.ensure(V * 4 + 4);
+      for (int k = 0; k < 2; ++k) {
+        moved |= small[k].ensure(V * 4 + 4);
+        moved |= medium[k].ensure(V * 4 + 4);
+        moved |= large[k].ensure((V + E / LCHUNK + 1) * 8);
+      }
+    }
+    if (moved) drop_graphs();
+  }
+
+  void ensure_vals(size_t elems) {
+    if (vals.ensure(elems * 4)) drop_graphs();
+  }
+
+  void stage(size_t ints) {
+    if (ints <= h_stage_cap) return;
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    HIP_CHECK(hipHostMalloc((void**)&h_stage, ints * 4, hipHostMallocDefault));
+    h_stage_cap = ints;
+  }
+
+  // ---- timing helpers (profile mode only)
+  void tick(int cat, bool begin) {
+    if (!opts.profile) return;
+    if (ev_used >= ev_pool.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    hipEvent_t e = ev_pool[ev_used++];
+    HIP_CHECK(hipEventRecord(e, s));
+    if (begin) ev.push_back({cat, {e, nullptr}});
+    else ev.back().second.second = e;
+  }
+};
+
+namespace {
+
+enum { CAT_INIT = 0, CAT_ADV = 1, CAT_FILTER = 2, CAT_WCC = 3, CAT_OUT = 4 };
+
+template <int W>
+void launch_iteration(cgp_engine* en, const KP& P, bool with_pull) {
+  hipStream_t s = en->s;
+  en->tick(CAT_ADV, true);
+  hipLaunchKernelGGL((k_push_large<W>), dim3(en->grid_large()), dim3(BLOCK), 0, s, P);
+  hipLaunchKernelGGL((k_push_group<W, 64>), dim3(en->grid_group()), dim3(BLOCK), 0, s, P, 1);
+  hipLaunchKernelGGL((k_push_group<W, 8>), dim3(en->grid_group()), dim3(BLOCK), 0, s, P, 0);
+  hipLaunchKernelGGL((k_push_dense<W>), dim3(en->grid_dense()), dim3(BLOCK), 0, s, P);
+  if (with_pull) hipLaunchKernelGGL((k_pull<W>), dim3(en->grid_pull()), dim3(BLOCK), 0, s, P);
+  en->tick(CAT_ADV, false);
+  en->tick(CAT_FILTER, true);
+  hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
+  en->tick(CAT_FILTER, false);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <int W>
+void launch_seed(cgp_engine* en, const KP& P, const int32_t* d_slot_src, int nslots) {
+  hipLaunchKernelGGL((k_seed<W>), dim3((nslots + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, en->s, P,
+                     d_slot_src, nslots);
+  hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, en->s, P);
+  HIP_CHECK(hipGetLastError());
+}
+
+void dispatch_iteration(cgp_engine* en, int W, const KP& P, bool with_pull) {
+  switch (W) {
+    case 1: launch_iteration<1>(en, P, with_pull); break;
+    case 2: launch_iteration<2>(en, P, with_pull); break;
+    case 4: launch_iteration<4>(en, P, with_pull); break;
+    default: launch_iteration<8>(en, P, with_pull); break;
+  }
+}
+
+void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* src, int nslots) {
+  switch (W) {
+    case 1: launch_seed<1>(en, P, src, nslots); break;
+    case 2: launch_seed<2>(en, P, src, nslots); break;
+    case 4: launch_seed<4>(en, P, src, nslots); break;
+    default: launch_seed<8>(en, P, src, nslots); break;
+  }
+}
+
+void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
+  if (n == 0) return;
+  size_t blocks = std::min<size_t>((n / 4 + BLOCK - 1) / BLOCK + 1, (size_t)cus * 16);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(BLOCK), 0, s, (u32*)p, n, v);
+  HIP_CHECK(hipGetLastError());
+}
+
+int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
+
+// Runs the frontier iterations of one pass to convergence.
+void run_iterations(cgp_engine* en, int W, bool with_pull) {
+  const KP P = en->kp();
+  const bool use_graph = en->opts.use_hip_graph && !en->opts.profile;
+  const int K = std::max(1, en->opts.graph_iterations);
+  if (use_graph) {
+    const int li = wlog(W);
+    if (!en->gexec[li] || en->gexec_iters[li] != K * (with_pull ? 1 : -1)) {
+      if (en->gexec[li]) (void)hipGraphExecDestroy(en->gexec[li]);
+      en->gexec[li] = nullptr;
+      hipGraph_t graph = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(en->s, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < K; ++k) dispatch_iteration(en, W, P, with_pull);
+      HIP_CHECK(hipStreamEndCapture(en->s, &graph));
+      hipError_t ie = hipGraphInstantiate(&en->gexec[li], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_CHECK(ie);
+      en->gexec_iters[li] = K * (with_pull ? 1 : -1);
+    }
+  }
+  for (;;) {
+    if (use_graph) {
+      HIP_CHECK(hipGraphLaunch(en->gexec[wlog(W)], en->s));
+    } else {
+      dispatch_iteration(en, W, P, with_pull);
+    }
+    HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
+    HIP_CHECK(hipStreamSynchronize(en->s));
+    if (en->h_ctl->finished) break;
+    // Frontier iterations are bounded by V + 1 (each one settles at least one
+    // more hop of some shortest path); more means a broken state machine.
+    if ((int64_t)en->h_ctl->iter > (int64_t)en->g->V + 2)
+      cgpi::fail(CGP_EDEVICE, "traversal did not converge within V+2 iterations");
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batch run
+namespace cgpi {
+
+struct PassPlan {
+  std::vector<int32_t> qi;  // user query index per slot
+  int nbfs = 0, nsssp = 0;
+};
+
+void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
+                      bool out_dev, cgp_run_stats* stats) {
+  const cgp_graph* g = en->g;
+  const int32_t V = g->V;
+  if (n <= 0) fail(CGP_EINVAL, "cgp_run_batch: batch must contain at least one query");
+  if (!queries || !out) fail(CGP_EINVAL, "cgp_run_batch: null queries or output");
+  std::vector<int32_t> bfs, sssp, wcc;
+  for (int32_t i = 0; i < n; ++i) {
+    const cgp_query& q = queries[i];
+    if (q.kind != CGP_BFS && q.kind != CGP_SSSP && q.kind != CGP_WCC)
+      fail(CGP_EINVAL, "cgp_run_batch: unknown query kind " + std::to_string(q.kind));
+    if (q.kind != CGP_WCC && (q.source < 0 || q.source >= V))
+      fail(CGP_EINVAL, "cgp_run_batch: query source out of range: " + std::to_string(q.source));
+    (q.kind == CGP_BFS ? bfs : q.kind == CGP_SSSP ? sssp : wcc).push_back(i);
+  }
+  cgp_run_stats st{};
+  if (V == 0) {
+    if (stats) *stats = st;
+    return;
+  }
+  en->ev.clear();
+  en->ev_used = 0;
+  const double t0 = now_ms();
+
+  // Output target: the caller's device buffer, or a cached device buffer.
+  u32* dout = out;
+  if (!out_dev) {
+    en->outbuf.ensure((size_t)n * V * 4);
+    dout = (u32*)en->outbuf.p;
+  }
+  if (en->opts.profile) HIP_CHECK(hipMemsetAsync(en->d_ctl, 0, sizeof(Ctl), en->s));
+
+  // Passes of at most max_words*64 slots; BFS slots first, then SSSP.
+  const int slots_max = std::max(1, std::min(en->opts.max_words, MAXW)) * 64;
+  std::vector<int32_t> order(bfs);
+  order.insert(order.end(), sssp.begin(), sssp.end());
+  std::vector<double> prof_sum(NPROF, 0.0);
+  for (size_t p0 = 0; p0 < order.size(); p0 += slots_max) {
+    const int ns = (int)std::min<size_t>(slots_max, order.size() - p0);
+    const int Wused = (ns + 63) / 64;
+    const int W = round_words(Wused);
+    int R = 1;
+    while (R < std::min(64, ns)) R <<= 1;
+    if (Wused > 1) R = 64;
+    int rshift = 0;
+    while ((1 << rshift) < R) ++rshift;
+    en->ensure_words(W);
+    en->ensure_vals((size_t)W * V * R);
+    const KP P = en->kp();
+    // Slot configuration.
+    Cfg& cfg = *en->h_cfg;
+    std::memset(&cfg, 0, sizeof(Cfg));
+    cfg.rshift = rshift;
+    cfg.W = W;
+    cfg.dense_t = 16;
+    en->stage((size_t)W * 64 * 2);
+    int32_t* h_src = en->h_stage;
+    int32_t* h_qmap = en->h_stage + W * 64;
+    int nb = 0;
+    for (int k = 0; k < W * 64; ++k) {
+      h_src[k] = -1;
+      h_qmap[k] = -1;
+    }
+    for (int k = 0; k < ns; ++k) {
+      const int32_t qi = order[p0 + k];
+      const int w = k >> 6, q = k & 63;
+      h_src[k] = queries[qi].source;
+      h_qmap[k] = qi;
+      if (queries[qi].kind == CGP_BFS) {
+        cfg.bfs_bits[w] |= 1ull << q;
+        ++nb;
+      } else {
+        cfg.val_bits[w] |= 1ull << q;
+      }
+    }
+    const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH;
+    cfg.pull_policy = !can_pull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
+    cfg.pull_pairs_thr = (u64)(en->opts.pull_frontier_ratio * (double)nb * (double)V);
+    cfg.pull_edges_thr = (u64)(en->opts.pull_edge_ratio * (double)nb * (double)g->E);
+    cfg.push_pairs_thr = cfg.pull_pairs_thr / 4 + 1;
+    if (cfg.pull_pairs_thr == 0) cfg.pull_pairs_thr = 1;
+    if (cfg.pull_edges_thr == 0) cfg.pull_edges_thr = 1;
+    // Control block.
+    Ctl& c = *en->h_ctl;
+    std::memset(&c, 0, sizeof(Ctl));
+    c.iter = 0xFFFFFFFFu;
+    if (en->stamp_epoch > 0xF0000000u) {
+      HIP_CHECK(hipMemsetAsync(en->mark.p, 0, (size_t)V * 4, en->s));
+      en->stamp_epoch = 1;
+    }
+    c.stamp_base = en->stamp_epoch;
+    c.max_iter = en->opts.max_iterations > 0 ? (u32)en->opts.max_iterations : 0u;
+    en->slot_src.ensure((size_t)W * 64 * 4);
+    en->qmap.ensure((size_t)W * 64 * 4);
+    en->tick(CAT_INIT, true);
+    // Profile counters are accumulated over passes: keep them on the device.
+    if (en->opts.profile) {
+      en->prof_out.ensure(sizeof(u64) * NPROF);
+      HIP_CHECK(hipMemcpyAsync(en->prof_out.p, &en->d_ctl->prof[0], sizeof(u64) * NPROF,
+                               hipMemcpyDeviceToDevice, en->s));
+    }
+    HIP_CHECK(hipMemcpyAsync(en->d_cfg, &cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
+    HIP_CHECK(hipMemcpyAsync(en->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, en->s));
+    if (en->opts.profile)
+      HIP_CHECK(hipMemcpyAsync(&en->d_ctl->prof[0], en->prof_out.p, sizeof(u64) * NPROF,
+                               hipMemcpyDeviceToDevice, en->s));
+    HIP_CHECK(hipMemcpyAsync(en->slot_src.p, h_src, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
+    HIP_CHECK(hipMemcpyAsync(en->qmap.p, h_qmap, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
+    fill_u32(en->s, en->vals.p, (size_t)W * V * R, INF, en->cus);
+    if (nb > 0) fill_u32(en->s, en->visited.p, (size_t)V * W * 2, 0u, en->cus);
+    if (en->next_dirty) {
+      fill_u32(en->s, en->next.p, (size_t)V * W * 2, 0u, en->cus);
+      en->next_dirty = false;
+    }
+    dispatch_seed(en, W, P, (const int32_t*)en->slot_src.p, W * 64);
+    en->tick(CAT_INIT, false);
+    run_iterations(en, W, can_pull);
+    const Ctl& hc = *en->h_ctl;
+    st.iterations += (int32_t)hc.iter;
+    st.push_iterations += (int32_t)hc.push_iters;
+    st.pull_iterations += (int32_t)hc.pull_iters;
+    st.passes += 1;
+    en->stamp_epoch += hc.iter + 2;
+    if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
+    en->tick(CAT_OUT, true);
+    hipLaunchKernelGGL(k_output, dim3((V + 63) / 64, Wused), dim3(BLOCK), 0, en->s,
+                       (const u32*)en->vals.p, V, rshift, (const int32_t*)en->qmap.p, dout);
+    HIP_CHECK(hipGetLastError());
+    en->tick(CAT_OUT, false);
+  }
+  if (!wcc.empty()) {
+    en->parent.ensure((size_t)V * 4 + 4);
+    en->wcc_rows.ensure(wcc.size() * 4);
+    en->stage(wcc.size());
+    std::memcpy(en->h_stage, wcc.data(), wcc.size() * 4);
+    HIP_CHECK(hipMemcpyAsync(en->wcc_rows.p, en->h_stage, wcc.size() * 4, hipMemcpyHostToDevice, en->s));
+    const int gb = en->cus * 16;
+    en->tick(CAT_WCC, true);
+    hipLaunchKernelGGL(k_wcc_init, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, V);
+    hipLaunchKernelGGL(k_wcc_hook, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->rows,
+                       (const int32_t*)g->idx, g->E, (int32_t*)en->parent.p, g->symmetric ? 1 : 0);
+    hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, V);
+    en->tick(CAT_WCC, false);
+    en->tick(CAT_OUT, true);
+    hipLaunchKernelGGL(k_wcc_rows, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p, V,
+                       (const int32_t*)en->wcc_rows.p, (int)wcc.size(), dout);
+    en->tick(CAT_OUT, false);
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(en->s));
+  st.wall_ms = now_ms() - t0;
+  if (!out_dev) {
+    HIP_CHECK(hipMemcpy(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost));
+  }
+  if (en->opts.profile) {
+    HIP_CHECK(hipMemcpy(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const u64* pf = en->h_ctl->prof;
+    for (auto& e : en->ev) {
+      float ms = 0;
+      if (e.second.second) HIP_CHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+      switch (e.first) {
+        case CAT_INIT: st.init_ms += ms; break;
+        case CAT_ADV: st.advance_ms += ms; st.advance_launches += 1; break;
+        case CAT_FILTER: st.filter_ms += ms; break;
+        case CAT_WCC: st.wcc_ms += ms; break;
+        default: st.output_ms += ms; break;
+      }
+    }
+    // Algorithmic bytes of the advance launches (each byte at its natural
+    // width, counted once):
+    //   frontier item: 8 (offset pair) + 8 W (frontier words)
+    //   edge scanned:  4 (column index) [+4 weight when SSSP bits are active]
+    //   (edge, BFS word): 8 (visited word of the target)
+    //   (edge, SSSP query): 4 (target value read) ; improved value: 4 (store)
+    //   next word gaining bits: 16 (read + write of the next-frontier word)
+    //   pull: 8 + 8 W per vertex examined, 4 per edge, 8 per (edge, word)
+    const double Wd = 8.0 * std::max(1, en->wcap);
+    st.advance_bytes = (double)pf[PF_PUSH_ITEMS] * (8.0 + Wd) + (double)pf[PF_PUSH_EDGES] * 4.0 +
+                       (double)pf[PF_PUSH_WEDGES] * 4.0 + (double)pf[PF_BFS_WORDS] * 8.0 +
+                       (double)pf[PF_SSSP_PAIRS] * 4.0 + (double)pf[PF_IMPROVED] * 4.0 +
+                       (double)pf[PF_NEW_WORDS] * 16.0 + (double)pf[PF_DENSE_ITEMS] * 16.0 +
+                       (double)pf[PF_DENSE_EDGES] * 8.0 + (double)pf[PF_PULL_VERTS] * (8.0 + Wd) +
+                       (double)pf[PF_PULL_EDGES] * 4.0 + (double)pf[PF_PULL_WORDS] * 8.0;
+    st.frontier_vertices = (int64_t)(pf[PF_PUSH_ITEMS] + pf[PF_DENSE_ITEMS]);
+    st.edges_scanned = (int64_t)(pf[PF_PUSH_EDGES] + pf[PF_DENSE_EDGES] + pf[PF_PULL_EDGES]);
+  }
+  if (stats) *stats = st;
+}
+
+}  // namespace cgpi
+
+// ================================================================= C-ABI (device side)
+#define CGP_TRY(expr)                                  \
+  try {                                                \
+    expr;                                              \
+    return CGP_OK;                                     \
+  } catch (const cgpi::Error& e_) {                    \
+    cgpi::set_last_error(e_.what());                   \
+    return e_.status;                                  \
+  } catch (const std::bad_alloc&) {                    \
+    cgpi::set_last_error("host allocation failed");    \
+    return CGP_ENOMEM;                                 \
+  } catch (const std::exception& e_) {                 \
+    cgpi::set_last_error(e_.what());                   \
+    return CGP_EINVAL;                                 \
+  }
+
+extern "C" {
+
+cgp_status cgp_device_count(int32_t* count) {
+  CGP_TRY({
+    if (!count) cgpi::fail(CGP_EINVAL, "cgp_device_count: null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+cgp_status cgp_device_info(char* name, int32_t name_cap, int32_t* compute_units, uint64_t* hbm_bytes) {
+  CGP_TRY({
+    require_device();
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, 0));
+    if (name && name_cap > 0) {
+      std::snprintf(name, (size_t)name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+  });
+}
+
+cgp_status cgp_device_alloc(uint64_t bytes, void** out) {
+  CGP_TRY({
+    if (!out) cgpi::fail(CGP_EINVAL, "cgp_device_alloc: null output");
+    require_device();
+    *out = nullptr;
+    HIP_CHECK(hipMalloc(out, bytes ? bytes : 1));
+  });
+}
+
+cgp_status cgp_device_free(void* ptr) {
+  CGP_TRY({
+    if (ptr) HIP_CHECK(hipFree(ptr));
+  });
+}
+
+cgp_status cgp_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+  CGP_TRY({ HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+cgp_status cgp_copy_to_device(void* dst, const void* src, uint64_t bytes) {
+  CGP_TRY({ HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,
+                                const uint32_t* wt, cgp_graph** out) {
+  CGP_TRY({
+    if (!out) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: null output");
+    *out = nullptr;
+    if (V < 0 || E < 0 || E >= INT32_MAX) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: bad sizes");
+    if (!off || (E > 0 && !idx)) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: null CSR array");
+    if (off[0] != 0 || off[V] != E) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: offsets do not span [0, E]");
+    for (int32_t v = 0; v < V; ++v)
+      if (off[v + 1] < off[v]) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: offsets not monotone");
+    for (int64_t e = 0; e < E; ++e)
+      if (idx[e] < 0 || idx[e] >= V) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: column index out of range");
+    require_device();
+    cgpi::HostCSR h;
+    h.V = V;
+    h.E = E;
+    h.off.assign(off, off + V + 1);
+    h.idx.assign(idx, idx + E);
+    if (wt) h.wt.assign(wt, wt + E);
+    std::unique_ptr<cgp_graph> g(new cgp_graph());
+    g->V = V;
+    g->E = E;
+    g->weighted = wt != nullptr;
+    g->symmetric = cgpi::is_symmetric(h);
+    HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
+    HIP_CHECK(hipMalloc((void**)&g->idx, (size_t)E * 4 + 4));
+    HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
+    if (wt) HIP_CHECK(hipMalloc((void**)&g->wt, (size_t)E * 4 + 4));
+    HIP_CHECK(hipMemcpy(g->off, off, ((size_t)V + 1) * 4, hipMemcpyHostToDevice));
+    if (E) HIP_CHECK(hipMemcpy(g->idx, idx, (size_t)E * 4, hipMemcpyHostToDevice));
+    if (wt && E) HIP_CHECK(hipMemcpy(g->wt, wt, (size_t)E * 4, hipMemcpyHostToDevice));
+    if (V > 0) {
+      int cus = device_cus();
+      hipLaunchKernelGGL(k_fill_rows, dim3(cus * 16), dim3(BLOCK), 0, 0, (const int32_t*)g->off, V, g->rows);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipDeviceSynchronize());
+    }
+    *out = g.release();
+  });
+}
+
+cgp_status cgp_graph_from_host(const cgp_host_csr* h, cgp_graph** out) {
+  if (!h) {
+    cgpi::set_last_error("cgp_graph_from_host: null host graph");
+    return CGP_EINVAL;
+  }
+  return cgp_graph_create_csr(h->g.V, h->g.E, h->g.off.data(), h->g.idx.data(),
+                              h->g.wt.empty() ? nullptr : h->g.wt.data(), out);
+}
+
+void cgp_graph_destroy(cgp_graph* g) {
+  if (!g) return;
+  (void)hipFree(g->off);
+  (void)hipFree(g->idx);
+  (void)hipFree(g->rows);
+  if (g->wt) (void)hipFree(g->wt);
+  delete g;
+}
+
+int32_t cgp_graph_num_vertices(const cgp_graph* g) { return g ? g->V : -1; }
+int64_t cgp_graph_num_edges(const cgp_graph* g) { return g ? g->E : -1; }
+int32_t cgp_graph_is_symmetric(const cgp_graph* g) { return g && g->symmetric ? 1 : 0; }
+
+void cgp_engine_opts_default(cgp_engine_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->traversal_mode = CGP_HYBRID;
+  o->pull_frontier_ratio = 0.15;
+  o->pull_edge_ratio = 0.20;
+  o->max_iterations = 0;
+  o->max_words = MAXW;
+  o->block_size = BLOCK;
+  o->bin_small = 16;
+  o->bin_large = 2048;
+  o->use_hip_graph = 1;
+  o->graph_iterations = 8;
+  o->profile = 0;
+  o->hbm_budget_bytes = 64ull << 30;
+}
+
+cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cgp_engine** out) {
+  CGP_TRY({
+    if (!g || !out) cgpi::fail(CGP_EINVAL, "cgp_engine_create: null graph or output");
+    *out = nullptr;
+    require_device();
+    std::unique_ptr<cgp_engine> en(new cgp_engine());
+    en->g = g;
+    cgp_engine_opts d;
+    cgp_engine_opts_default(&d);
+    if (opts) {
+      en->opts = *opts;
+      if (en->opts.pull_frontier_ratio <= 0) en->opts.pull_frontier_ratio = d.pull_frontier_ratio;
+      if (en->opts.pull_edge_ratio <= 0) en->opts.pull_edge_ratio = d.pull_edge_ratio;
+      if (en->opts.max_words <= 0 || en->opts.max_words > MAXW) en->opts.max_words = d.max_words;
+      if (en->opts.block_size <= 0) en->opts.block_size = d.block_size;
+      if (en->opts.bin_small <= 0) en->opts.bin_small = d.bin_small;
+      if (en->opts.bin_large <= 0) en->opts.bin_large = d.bin_large;
+      if (en->opts.graph_iterations <= 0) en->opts.graph_iterations = d.graph_iterations;
+      if (en->opts.hbm_budget_bytes == 0) en->opts.hbm_budget_bytes = d.hbm_budget_bytes;
+      if (en->opts.traversal_mode < 0 || en->opts.traversal_mode > 2)
+        cgpi::fail(CGP_EINVAL, "cgp_engine_create: bad traversal_mode");
+    } else {
+      en->opts = d;
+    }
+    if (en->opts.bin_large < en->opts.bin_small) en->opts.bin_large = en->opts.bin_small;
+    en->opts.block_size = BLOCK;
+    en->cus = device_cus();
+    HIP_CHECK(hipStreamCreateWithFlags(&en->s, hipStreamNonBlocking));
+    HIP_CHECK(hipMalloc((void**)&en->d_ctl, sizeof(Ctl)));
+    HIP_CHECK(hipMalloc((void**)&en->d_cfg, sizeof(Cfg)));
+    HIP_CHECK(hipHostMalloc((void**)&en->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&en->h_cfg, sizeof(Cfg), hipHostMallocDefault));
+    HIP_CHECK(hipMemset(en->d_ctl, 0, sizeof(Ctl)));
+    *out = en.release();
+  });
+}
+
+void cgp_engine_destroy(cgp_engine* en) {
+  if (!en) return;
+  if (en->s) (void)hipStreamSynchronize(en->s);
+  en->drop_graphs();
+  DevBuf* bufs[] = {&en->visited, &en->next, &en->front, &en->mark, &en->vals, &en->touched,
+                    &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
+                    &en->large[1], &en->dense[0], &en->dense[1], &en->parent, &en->wcc_rows,
+                    &en->outbuf, &en->qmap, &en->slot_src, &en->prof_out};
+  for (DevBuf* b : bufs) b->release();
+  for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
+  if (en->d_ctl) (void)hipFree(en->d_ctl);
+  if (en->d_cfg) (void)hipFree(en->d_cfg);
+  if (en->h_ctl) (void)hipHostFree(en->h_ctl);
+  if (en->h_cfg) (void)hipHostFree(en->h_cfg);
+  if (en->h_stage) (void)hipHostFree(en->h_stage);
+  if (en->s) (void)hipStreamDestroy(en->s);
+  delete en;
+}
+
+cgp_status cgp_engine_get_opts(const cgp_engine* en, cgp_engine_opts* out) {
+  CGP_TRY({
+    if (!en || !out) cgpi::fail(CGP_EINVAL, "cgp_engine_get_opts: null argument");
+    *out = en->opts;
+  });
+}
+
+cgp_status cgp_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
+                         int32_t out_on_device, cgp_run_stats* stats) {
+  CGP_TRY({
+    if (!en) cgpi::fail(CGP_EINVAL, "cgp_run_batch: null engine");
+    std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
+    if (!lk.owns_lock() || en->serve) cgpi::fail(CGP_EBUSY, "cgp_run_batch: engine is busy");
+    cgpi::engine_run_batch(en, queries, n, out, out_on_device != 0, stats);
+  });
+}
+
+cgp_status cgp_reached_edges(const cgp_graph* g, const uint32_t* out_dev, int32_t n, uint64_t* m_out) {
+  CGP_TRY({
+    if (!g || !out_dev || !m_out || n <= 0) cgpi::fail(CGP_EINVAL, "cgp_reached_edges: bad argument");
+    require_device();
+    u64* dm = nullptr;
+    HIP_CHECK(hipMalloc((void**)&dm, sizeof(u64) * (size_t)n));
+    HIP_CHECK(hipMemset(dm, 0, sizeof(u64) * (size_t)n));
+    const int cus = device_cus();
+    hipLaunchKernelGGL(k_reached, dim3(cus * 2, n), dim3(BLOCK), 0, 0, (const int32_t*)g->off, g->V, out_dev, dm);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(m_out, dm, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost);
+    (void)hipFree(dm);
+    HIP_CHECK(e);
+  });
+}
+
+}  // extern "C"

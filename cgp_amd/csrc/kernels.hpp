@@ -1,0 +1,120 @@
+// kernels.hpp — device-side data layout and HIP kernels of the concurrent
+// multi-query traversal engine (gfx950, wave64).
+//
+// Layout in HBM (V vertices, W mask words per vertex, R lanes per value row):
+//   off[V+1], idx[E] int32; wt[E] uint32 (absent = unit weights); rows[E] int32
+//     (source vertex of each CSR slot, built once per graph for edge-parallel WCC)
+//   visited/next/front  uint64[V*W]   vertex-major, word w of vertex v at v*W+w
+//   vals  uint32[W*V*R]                value of slot (w, q) at vertex v at
+//                                      ((w*V + v) << log2 R) + q: the R query
+//                                      values of one word of one vertex are one
+//                                      contiguous row, so a wave whose lanes are
+//                                      queries touches one 256 B row per edge.
+//   mark  uint32[V]                    iteration stamp: vertex already appended
+//                                      to the touched list this iteration
+//   touched int32[V]                   vertices whose next words gained bits
+//   small/medium int32[V], large uint64[V+E/LCHUNK+1], dense uint64[W*(V+E/DCHUNK+1)]
+//                                      degree-binned frontier work lists,
+//                                      double-buffered by iteration parity
+//
+// One iteration = push kernels (lane-per-edge over the three degree bins for
+// BFS bits and sparse SSSP words; lane-per-query waves for dense SSSP words),
+// an optional BFS pull kernel, then the filter kernel that claims next words,
+// settles BFS depths, and emits the next binned frontier with wave-level
+// prefix sums.  Every kernel reads its work counts from the device control
+// block, so a sequence of iterations can be captured in a hipGraph and
+// replayed without host round trips; the filter's last workgroup advances the
+// iteration and raises `finished` when the frontier is empty.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cgpk {
+
+using u32 = uint32_t;
+using u64 = unsigned long long;
+
+constexpr int MAXW = 8;          // mask words per pass (512 slots)
+constexpr u32 INF = 0xFFFFFFFFu;
+constexpr int BLOCK = 256;       // workgroup size of every kernel (4 waves)
+constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
+constexpr int DCHUNK = 512;      // edges per dense lane-per-query wave item
+constexpr int NPROF = 12;
+
+// Per-batch slot configuration (device resident, rewritten per pass).
+struct Cfg {
+  u64 bfs_bits[MAXW];   // slots settled by first-write BFS (visited bitmask)
+  u64 val_bits[MAXW];   // slots relaxed by atomicMin on values (SSSP)
+  int32_t start[MAXW * 64];  // iteration at which each slot's query started
+  int32_t rshift;       // log2(R)
+  int32_t W;            // words in use
+  int32_t pull_policy;  // 0 push only, 1 BFS pull whenever BFS bits are active, 2 thresholds
+  int32_t dense_t;      // popcount at which an SSSP word goes lane-per-query
+  u64 pull_pairs_thr;   // hybrid: pull when BFS frontier (vertex,query) pairs >= this
+  u64 pull_edges_thr;   // ... or when their edge-weighted count >= this
+  u64 push_pairs_thr;   // back to push when pairs fall below this
+};
+
+// Device control block: counters and the iteration state machine.
+struct Ctl {
+  u32 iter;         // iteration of the current run (0xFFFFFFFF during seeding)
+  u32 stamp_base;   // mark stamp of iteration i is stamp_base + i + 1
+  u32 finished;     // set when the next frontier is empty or max_iter reached
+  u32 bfs_pull;     // BFS bits use the pull kernel this iteration
+  u32 tcount;       // touched-list length
+  u32 done_blocks;  // filter last-workgroup detection
+  u32 max_iter;     // 0 = unlimited
+  u32 push_iters;
+  u32 pull_iters;
+  u32 pad[3];
+  u32 bcount[2][4];           // [parity][small, medium, large items, dense items]
+  u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
+  u64 bfs_edges[2];           // ... weighted by degree
+  u64 union_[MAXW];           // OR of next words in the running filter
+  u64 last_union[MAXW];       // OR of the frontier words the last filter produced
+  u64 prof[NPROF];            // work counters (profile mode)
+};
+
+// Static per-engine kernel parameters (buffer addresses and tuning).
+struct KP {
+  int32_t V;
+  int32_t profile;
+  int64_t E;
+  const int32_t* off;
+  const int32_t* idx;
+  const u32* wt;
+  const int32_t* rows;
+  u64* visited;
+  u64* next;
+  u64* front;
+  u32* mark;
+  u32* vals;
+  int32_t* touched;
+  int32_t* small[2];
+  int32_t* medium[2];
+  u64* large[2];
+  u64* dense[2];
+  Ctl* ctl;
+  const Cfg* cfg;
+  int32_t bin_small;
+  int32_t bin_large;
+};
+
+// Profile counter slots.
+enum : int {
+  PF_PUSH_ITEMS = 0,   // frontier items expanded by push (vertex or chunk)
+  PF_PUSH_EDGES,       // edges scanned lane-per-edge
+  PF_PUSH_WEDGES,      // ... of which read a weight
+  PF_BFS_WORDS,        // (edge, active BFS word) visited reads
+  PF_SSSP_PAIRS,       // (edge, active SSSP query) value reads
+  PF_NEW_WORDS,        // next words that gained bits (atomicOr that changed them)
+  PF_IMPROVED,         // (vertex, query) values lowered
+  PF_DENSE_ITEMS,      // dense (vertex, word, chunk) items
+  PF_DENSE_EDGES,      // edges scanned lane-per-query
+  PF_PULL_VERTS,       // vertices examined by pull
+  PF_PULL_WORDS,       // (edge, word) visited reads by pull
+  PF_PULL_EDGES        // edges scanned by pull
+};
+
+}  // namespace cgpk

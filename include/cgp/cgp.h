@@ -1,0 +1,229 @@
+/*
+ * cgp.h — C-ABI of the MI355X-native concurrent multi-query graph engine.
+ *
+ * One traversal of a device-resident CSR graph serves a whole batch of BFS,
+ * SSSP and WCC queries: every vertex carries W 64-bit query-mask words (one
+ * bit per query slot, the wave64 __ballot width), BFS bits are settled with
+ * one 64-bit atomicOr per (edge, word), SSSP values relax with native uint32
+ * atomicMin, and WCC is computed by hooking plus pointer jumping.
+ *
+ * The reference exposes this path only as header-only C++ templates; the
+ * entry points below are what its FFI for the path would bind.  Each one
+ * names the reference interface it replaces (paths relative to the
+ * reference root).
+ *
+ * Conventions (all entry points):
+ *   - C linkage, plain pointers and sizes, no exceptions cross the ABI.
+ *   - Every call returns a cgp_status; on failure cgp_last_error() returns a
+ *     thread-local message of the form "<op>: <reason>", the text the
+ *     reference puts into its std::runtime_error / std::invalid_argument
+ *     (puercgp/include/puercgp/engine/frontier_engine.hxx:27-32,
+ *      puercgp/include/puercgp/core/query_batch.hxx:44-56).
+ *   - Results are uint32 per (query, vertex), query-major out[q*V + v]
+ *     (cgp/include/gunrock/algorithms/cgp_bfs.hxx:133,214), with
+ *     CGP_INF = 0xFFFFFFFF for "unreachable".
+ *   - An engine owns one HIP stream of default priority; calls on one engine
+ *     are serialised by the caller (as with one reference execution_context,
+ *     puercgp/include/puercgp/backend/graph_adapter.hxx:43-97).
+ */
+#ifndef CGP_CGP_H
+#define CGP_CGP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGP_ABI_VERSION 1
+#define CGP_INF 0xFFFFFFFFu
+
+typedef enum {
+  CGP_OK = 0,
+  CGP_EINVAL = 1,     /* bad argument or batch (reference: std::invalid_argument) */
+  CGP_ENOMEM = 2,     /* device or host allocation failed */
+  CGP_EDEVICE = 3,    /* HIP runtime error or no usable GPU (reference: std::runtime_error) */
+  CGP_EBUSY = 4,      /* engine already running a batch / serving session */
+  CGP_ECAPACITY = 5,  /* request exceeds slots or HBM budget */
+  CGP_EIO = 6         /* graph file could not be read or parsed */
+} cgp_status;
+
+/* Same ordinals as algo_kind_t (puercgp/include/puercgp/algorithms/hybrid.hxx:21-26). */
+typedef enum { CGP_BFS = 0, CGP_SSSP = 1, CGP_WCC = 2 } cgp_kind;
+
+/* Replaces query_descriptor_t{source, kind} (puercgp/include/puercgp/core/query_descriptor.hxx:19-24).
+ * WCC ignores source. */
+typedef struct {
+  int32_t source;
+  int32_t kind;
+} cgp_query;
+
+/* Traversal direction policy; replaces traversal_mode_t (puercgp/include/puercgp/core/types.hxx:10). */
+typedef enum { CGP_PUSH = 0, CGP_PULL = 1, CGP_HYBRID = 2 } cgp_traversal_mode;
+
+typedef struct cgp_graph cgp_graph;        /* device-resident CSR */
+typedef struct cgp_host_csr cgp_host_csr;  /* host CSR produced by a generator or loader */
+typedef struct cgp_engine cgp_engine;      /* per-graph batch state, stream, tuned parameters */
+
+/* Engine options.  Replaces run_options (puercgp/include/puercgp/core/types.hxx:26-46)
+ * and cgp_bfs::options_t (cgp/include/gunrock/algorithms/cgp_bfs.hxx:97-105).
+ * Zero in a tunable field means "use the tuned / default value". */
+typedef struct {
+  int32_t traversal_mode;      /* cgp_traversal_mode; default CGP_HYBRID */
+  double pull_frontier_ratio;  /* pull when frontier >= ratio * V (default 0.15, types.hxx:30) */
+  double pull_edge_ratio;      /* pull when frontier edges >= ratio * E (default 0.20, types.hxx:31) */
+  int32_t max_iterations;      /* 0 = until convergence (types.hxx:36) */
+  int32_t max_words;           /* cap on 64-bit mask words per pass; 0 = tuned */
+  int32_t block_size;          /* workgroup size of the advance kernels (64..1024); 0 = tuned */
+  int32_t bin_small;           /* degree <= bin_small: 8 lanes per vertex; 0 = tuned */
+  int32_t bin_large;           /* degree > bin_large: one workgroup per vertex; 0 = tuned */
+  int32_t use_hip_graph;       /* capture iterations into a hipGraph (default 1) */
+  int32_t graph_iterations;    /* iterations per captured graph launch (default 8) */
+  int32_t profile;             /* 1 = per-launch HIP events and device work counters */
+  uint64_t hbm_budget_bytes;   /* serving: cap on per-query value memory; 0 = 64 GiB */
+} cgp_engine_opts;
+
+/* Per-run report; replaces run_result_t / iteration_profile_t
+ * (puercgp/include/puercgp/core/result.hxx:13-66) and cgp_bfs::result_t timing fields. */
+typedef struct {
+  int32_t iterations;          /* frontier iterations executed */
+  int32_t push_iterations;
+  int32_t pull_iterations;
+  int32_t passes;              /* word-group passes (batch split by max_words) */
+  double wall_ms;              /* host wall, first launch -> results resident on the device */
+  double init_ms;              /* profile only: init kernels */
+  double advance_ms;           /* profile only: summed push+pull advance kernel time */
+  double filter_ms;            /* profile only: summed compaction kernel time */
+  double wcc_ms;               /* profile only: hooking + pointer jumping */
+  double output_ms;            /* profile only: result transpose */
+  int64_t advance_launches;    /* profile only: advance launches that did work */
+  double advance_bytes;        /* profile only: algorithmic bytes of all advance launches */
+  int64_t frontier_vertices;   /* profile only: summed frontier sizes */
+  int64_t edges_scanned;       /* profile only: edges touched by push and pull */
+} cgp_run_stats;
+
+/* ---------------------------------------------------------------- errors */
+const char* cgp_last_error(void);
+int32_t cgp_abi_version(void);
+
+/* ---------------------------------------------------------------- device */
+/* Number of visible HIP devices (0 on a host without a usable GPU). */
+cgp_status cgp_device_count(int32_t* count);
+/* Name and compute-unit count of device 0. */
+cgp_status cgp_device_info(char* name, int32_t name_cap, int32_t* compute_units, uint64_t* hbm_bytes);
+/* Plain device buffers for callers without a device allocator of their own. */
+cgp_status cgp_device_alloc(uint64_t bytes, void** out);
+cgp_status cgp_device_free(void* ptr);
+cgp_status cgp_copy_to_host(void* dst_host, const void* src_device, uint64_t bytes);
+cgp_status cgp_copy_to_device(void* dst_device, const void* src_host, uint64_t bytes);
+
+/* ---------------------------------------------------------------- host CSR: generators and loaders */
+/* RMAT (a,b,c, d = 1-a-b-c) with 2^scale vertices and edge_factor * 2^scale edge draws,
+ * self-loops dropped, symmetrised, deduplicated, sorted; weights uniform in
+ * [wmin, wmax] from a hash of (min(u,v), max(u,v), seed) so both directions agree.
+ * wmax == 0 produces an unweighted graph. Has no reference counterpart. */
+cgp_status cgp_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                        uint64_t seed, uint32_t wmin, uint32_t wmax, cgp_host_csr** out);
+/* rows x cols grid, 4-neighbour undirected edges, plus per vertex with probability
+ * p_extra one undirected edge to a uniformly chosen vertex. */
+cgp_status cgp_gen_grid(int32_t rows, int32_t cols, double p_extra, uint64_t seed,
+                        uint32_t wmin, uint32_t wmax, cgp_host_csr** out);
+/* Build a host CSR from a COO edge list (sorted; duplicates kept, as csr_t::from_coo,
+ * cgp/include/gunrock/formats/csr.hxx:81-140). */
+cgp_status cgp_host_csr_from_coo(int32_t n_vertices, int64_t n_edges, const int32_t* src,
+                                 const int32_t* dst, const uint32_t* weights, cgp_host_csr** out);
+/* Loaders (reference: load_gr cgp/examples/algorithms/bfs/cgp_bfs.cu:214-275,
+ * load_matrix_market puercgp/examples/mtx_loader.hxx:99-187,
+ * load_binary_csr_directory puercgp/examples/mtx_loader.hxx:189-260). */
+cgp_status cgp_load_gr(const char* path, cgp_host_csr** out);
+cgp_status cgp_load_matrix_market(const char* path, cgp_host_csr** out);
+cgp_status cgp_load_csr_directory(const char* path, cgp_host_csr** out);
+/* Borrow the arrays of a host CSR (valid until cgp_host_csr_destroy). weights may be NULL. */
+cgp_status cgp_host_csr_view(const cgp_host_csr* g, int32_t* n_vertices, int64_t* n_edges,
+                             const int32_t** row_offsets, const int32_t** col_indices,
+                             const uint32_t** weights);
+void cgp_host_csr_destroy(cgp_host_csr* g);
+/* n distinct sources drawn uniformly from vertices with out-degree >= 1. */
+cgp_status cgp_sample_sources(const cgp_host_csr* g, int32_t n, uint64_t seed, int32_t* out);
+/* Open-loop Poisson arrivals: n exponential inter-arrival gaps of rate `rate_per_s`,
+ * cumulative, in seconds. */
+cgp_status cgp_poisson_arrivals(int32_t n, double rate_per_s, uint64_t seed, double* out_s);
+
+/* ---------------------------------------------------------------- device graph (A1) */
+/* Replaces csr_graph_storage / graph::build (puercgp/include/puercgp/backend/csr_graph.hxx:9-75,
+ * cgp/include/gunrock/graph/build.hxx:28-35). Copies host CSR into HBM.
+ * weights == NULL means unit weights (csr_graph.hxx:37-39). */
+cgp_status cgp_graph_create_csr(int32_t n_vertices, int64_t n_edges, const int32_t* row_offsets,
+                                const int32_t* col_indices, const uint32_t* weights,
+                                cgp_graph** out);
+cgp_status cgp_graph_from_host(const cgp_host_csr* g, cgp_graph** out);
+void cgp_graph_destroy(cgp_graph* g);
+int32_t cgp_graph_num_vertices(const cgp_graph* g);
+int64_t cgp_graph_num_edges(const cgp_graph* g);
+/* 1 when every edge u->v has a matching v->u (pull then reads the out-rows). */
+int32_t cgp_graph_is_symmetric(const cgp_graph* g);
+
+/* ---------------------------------------------------------------- engine */
+void cgp_engine_opts_default(cgp_engine_opts* opts);
+cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cgp_engine** out);
+void cgp_engine_destroy(cgp_engine* e);
+/* Options actually in force (after defaults and tuning). */
+cgp_status cgp_engine_get_opts(const cgp_engine* e, cgp_engine_opts* out);
+
+/* Batched run of n queries of any mix of kinds.
+ * Replaces gunrock::cgp_bfs::run / cgp_sssp::run (cgp/include/gunrock/algorithms/cgp_bfs.hxx:1050-1056,
+ * cgp_sssp.hxx:973-979), puercgp::run<Policy> (puercgp/include/puercgp/run.hxx:11-23) and
+ * run_heterogeneous (puercgp/include/puercgp/engine/hybrid_engine.hxx:1144-1150).
+ * out: n*V uint32, query-major; out_on_device != 0 means `out` is a device pointer,
+ * otherwise a host pointer (results are copied back before return). stats may be NULL. */
+cgp_status cgp_run_batch(cgp_engine* e, const cgp_query* queries, int32_t n, uint32_t* out,
+                         int32_t out_on_device, cgp_run_stats* stats);
+
+/* Per-query edge counts m_q = sum of out-degrees of the vertices reached by query q
+ * (out[q*V+v] != CGP_INF); device-resident results in, n counts out (host). */
+cgp_status cgp_reached_edges(const cgp_graph* g, const uint32_t* out_device, int32_t n,
+                             uint64_t* m_out);
+
+/* Tune bitmask word count, degree-bin thresholds and workgroup size for this graph
+ * on a sample batch (n queries); results stick to the engine and are returned in `chosen`. */
+cgp_status cgp_engine_autotune(cgp_engine* e, const cgp_query* sample, int32_t n,
+                               cgp_engine_opts* chosen);
+
+/* ---------------------------------------------------------------- serving (continuous batching) */
+/* Replaces run_replenish_pipeline (puercgp/include/puercgp/engine/replenish_engine.hxx:559-573). */
+typedef struct {
+  int32_t max_slots;        /* bitmask slots kept resident (multiple of 64); 0 = from budget */
+  int32_t keep_results;     /* 1 = completions carry the V-length result row (host copy) */
+} cgp_serve_opts;
+
+typedef struct {
+  int64_t ticket;           /* caller's id from cgp_serve_submit */
+  int32_t kind;
+  int32_t source;
+  double arrival_s;         /* caller clock, seconds (as submitted) */
+  double admit_s;           /* engine clock when the query got a slot */
+  double done_s;            /* engine clock when its result was resident on the host */
+  int32_t iterations;       /* frontier iterations the query took */
+  uint32_t* result;         /* V values when keep_results, else NULL; owned by the engine
+                               until cgp_serve_release_result */
+} cgp_completion;
+
+cgp_status cgp_serve_start(cgp_engine* e, const cgp_serve_opts* opts);
+/* Enqueue one query; it is admitted into a free slot at the next iteration boundary
+ * once its value arrays fit the HBM budget. Non-blocking. */
+cgp_status cgp_serve_submit(cgp_engine* e, const cgp_query* q, int64_t ticket, double arrival_s);
+/* Drive the device: run iterations until at least one query completes or
+ * `max_iterations` iterations ran (0 = until idle). Collects up to cap completions. */
+cgp_status cgp_serve_step(cgp_engine* e, int32_t max_iterations, cgp_completion* done,
+                          int32_t cap, int32_t* n_done);
+/* Engine clock (seconds since cgp_serve_start), same clock as admit_s / done_s. */
+double cgp_serve_clock(const cgp_engine* e);
+cgp_status cgp_serve_pending(const cgp_engine* e, int32_t* queued, int32_t* running);
+void cgp_serve_release_result(uint32_t* result);
+cgp_status cgp_serve_stop(cgp_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CGP_CGP_H */

@@ -1,0 +1,83 @@
+"""The C-ABI library loads, exports every entry point include/cgp/*.h
+declares, and reports errors the way the header documents (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import cgp_amd
+from cgp_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cgp", "cgp.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(cgp_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ["cgp_graph_create_csr", "cgp_engine_create", "cgp_run_batch", "cgp_serve_submit",
+                 "cgp_serve_step", "cgp_last_error", "cgp_engine_autotune", "cgp_load_gr"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_abi_version():
+    assert _lib.lib.cgp_abi_version() == 1
+
+
+def test_device_entry_points_fail_cleanly_without_gpu():
+    if cgp_amd.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    h = cgp_amd.HostCSR.rmat(6, 4, seed=1)
+    with pytest.raises(cgp_amd.CgpError) as e:
+        cgp_amd.Graph.from_host(h)
+    assert e.value.status == cgp_amd.CGP_EDEVICE
+    assert "HIP device" in e.value.message
+
+
+def test_bad_csr_rejected_before_touching_device():
+    with pytest.raises(cgp_amd.CgpError) as e:
+        cgp_amd.Graph.from_csr([0, 2, 1], [0, 1])
+    assert e.value.status == cgp_amd.CGP_EINVAL
+    with pytest.raises(cgp_amd.CgpError) as e:
+        cgp_amd.Graph.from_csr([0, 1, 2], [0, 5])
+    assert e.value.status == cgp_amd.CGP_EINVAL
+
+
+def test_last_error_is_thread_local():
+    import threading
+    msgs = {}
+
+    def worker(key, scale):
+        try:
+            cgp_amd.HostCSR.rmat(scale, 4)
+        except cgp_amd.CgpError as e:
+            msgs[key] = e.message
+
+    t = threading.Thread(target=worker, args=("t", 0))
+    t.start()
+    t.join()
+    assert "scale" in msgs["t"]
+
+
+def test_cpp_facade_compiles():
+    src = os.path.join(ROOT, "include", "cgp", "cgp.hpp")
+    if not os.path.exists(src):
+        pytest.skip("no C++ facade")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", src], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr

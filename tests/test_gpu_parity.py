@@ -1,0 +1,229 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle,
+bit-exact, on the reference's golden vectors and on seeded graphs."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cgp_amd
+import oracle_lib as O
+from cgp_amd import BFS, SSSP, WCC, INF
+from checks import check_bfs, check_sssp, check_wcc
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def toy(name):
+    t = GOLD["graphs"][name]
+    return cgp_amd.Graph.from_csr(t["offsets"], t["indices"], t["weights"]), O.G(t["offsets"], t["indices"], t["weights"])
+
+
+_cache = {}
+
+
+def rmat(scale, ef, seed, wmin=1, wmax=64):
+    key = (scale, ef, seed, wmin, wmax)
+    if key not in _cache:
+        h = cgp_amd.HostCSR.rmat(scale, ef, seed=seed, wmin=wmin, wmax=wmax)
+        _cache[key] = (h, cgp_amd.Graph.from_host(h), O.G.of(h))
+    return _cache[key]
+
+
+def oracle_rows(g, kinds, sources):
+    return np.stack([O.expected(g, k, s) for k, s in zip(kinds, sources)])
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=lambda c: c["name"])
+@pytest.mark.parametrize("mode", [cgp_amd.PUSH, cgp_amd.HYBRID])
+def test_golden_toy(gpu, case, mode):
+    say("golden", case["name"], "mode", mode)
+    g, _ = toy(case["graph"])
+    kind = {"bfs": BFS, "sssp": SSSP, "wcc": WCC}[case["kind"]]
+    en = cgp_amd.Engine(g, traversal_mode=mode, max_iterations=case.get("max_iterations", 0))
+    got = en.run_batch([kind], [case["source"]])[0][0]
+    assert got.tolist() == case["expect"], case["cite"]
+
+
+def test_golden_mixed_toy_batches(gpu):
+    # validate_hybrid.cu: BFS + SSSP + WCC in one batch on T1; validate_replenish.cu's
+    # four queries on T2 in one batch.
+    g, _ = toy("T1")
+    out = cgp_amd.Engine(g).run_batch([BFS, SSSP, WCC], [0, 0, 0])[0]
+    assert out.tolist() == [[0, 1, 1, 2], [0, 2, 5, 3], [0, 0, 0, 0]]
+    g, _ = toy("T2")
+    out = cgp_amd.Engine(g).run_batch([BFS, SSSP, BFS, SSSP], [0, 0, 1, 1])[0]
+    assert out.tolist() == [[0, 1, 1, 2], [0, 2, 5, 5], [INF, 0, INF, 1], [INF, 0, INF, 3]]
+
+
+@pytest.mark.parametrize("mm", GOLD["mm_cases"], ids=lambda c: c["file"])
+def test_golden_matrix_market(gpu, mm):
+    h = cgp_amd.HostCSR.load_matrix_market(os.path.join(HERE, "golden", mm["file"]))
+    g = cgp_amd.Graph.from_host(h)
+    kind = BFS if mm["kind"] == "bfs" else SSSP
+    srcs = sorted(int(s) for s in mm["expect"])
+    out = cgp_amd.Engine(g).run_batch([kind] * len(srcs), srcs)[0]
+    for i, s in enumerate(srcs):
+        assert out[i].tolist() == mm["expect"][str(s)], mm["cite"]
+
+
+@pytest.mark.parametrize("mode", [cgp_amd.PUSH, cgp_amd.PULL, cgp_amd.HYBRID])
+@pytest.mark.parametrize("nq", [1, 7, 64, 130])
+def test_rmat_bfs_batches(gpu, mode, nq):
+    say("rmat bfs mode", mode, "nq", nq)
+    h, g, og = rmat(12, 16, 3)
+    src = h.sample_sources(nq, seed=nq)
+    out = cgp_amd.Engine(g, traversal_mode=mode).run_batch([BFS] * nq, src)[0]
+    assert np.array_equal(out, oracle_rows(og, [0] * nq, src))
+
+
+@pytest.mark.parametrize("nq", [1, 5, 16, 64, 100])
+def test_rmat_sssp_batches(gpu, nq):
+    say("rmat sssp nq", nq)
+    h, g, og = rmat(12, 16, 4, 1, 255)
+    src = h.sample_sources(nq, seed=nq + 1)
+    out = cgp_amd.Engine(g).run_batch([SSSP] * nq, src)[0]
+    assert np.array_equal(out, oracle_rows(og, [1] * nq, src))
+
+
+def test_rmat_wcc(gpu):
+    h, g, og = rmat(13, 4, 5)
+    out = cgp_amd.Engine(g).run_batch([WCC] * 3, [0] * 3)[0]
+    exp = O.wcc_uf(og)
+    for r in out:
+        assert np.array_equal(r, exp)
+
+
+@pytest.mark.parametrize("hip_graph", [0, 1])
+@pytest.mark.parametrize("profile", [0, 1])
+def test_mixed_uneven_batch(gpu, hip_graph, profile):
+    # 160 BFS + 80 SSSP + 16 WCC in one batch (4 mask words, a mixed word),
+    # shuffled order; results must equal the oracle row by row.
+    say("mixed hip_graph", hip_graph, "profile", profile)
+    h, g, og = rmat(12, 16, 6, 1, 255)
+    src = h.sample_sources(240, seed=3)
+    kinds = [BFS] * 160 + [SSSP] * 80 + [WCC] * 16
+    sources = list(src) + [0] * 16
+    perm = np.random.default_rng(1).permutation(len(kinds))
+    kinds = [kinds[i] for i in perm]
+    sources = [int(sources[i]) for i in perm]
+    en = cgp_amd.Engine(g, use_hip_graph=hip_graph, profile=profile)
+    out, st = en.run_batch(kinds, sources)
+    exp = oracle_rows(og, kinds, sources)
+    assert np.array_equal(out, exp)
+    assert st["passes"] == 1
+    if profile:
+        assert st["advance_ms"] > 0 and st["advance_bytes"] > 0 and st["edges_scanned"] > 0
+
+
+def test_alone_equals_in_the_mix(gpu):
+    # Each query's result is identical run alone and inside a mixed batch.
+    h, g, og = rmat(11, 16, 8, 1, 100)
+    src = h.sample_sources(40, seed=5)
+    kinds = [BFS, SSSP] * 20
+    en = cgp_amd.Engine(g)
+    mix = en.run_batch(kinds, src)[0]
+    for i in range(0, 40, 7):
+        alone = en.run_batch([kinds[i]], [src[i]])[0][0]
+        assert np.array_equal(alone, mix[i])
+
+
+def test_multi_pass_word_split(gpu):
+    # max_words=1 splits 150 queries into three 64-slot passes.
+    h, g, og = rmat(11, 8, 9, 1, 30)
+    src = h.sample_sources(150, seed=2)
+    kinds = [BFS if i % 3 else SSSP for i in range(150)]
+    out, st = cgp_amd.Engine(g, max_words=1).run_batch(kinds, src)
+    assert st["passes"] == 3
+    assert np.array_equal(out, oracle_rows(og, kinds, src))
+
+
+def test_repeat_and_permutation_determinism(gpu):
+    h, g, og = rmat(11, 16, 10, 1, 255)
+    src = h.sample_sources(70, seed=7)
+    kinds = [SSSP] * 35 + [BFS] * 35
+    en = cgp_amd.Engine(g)
+    a = en.run_batch(kinds, src)[0]
+    b = en.run_batch(kinds, src)[0]
+    perm = np.arange(70)[::-1]
+    c = en.run_batch([kinds[i] for i in perm], [src[i] for i in perm])[0]
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c[perm])
+
+
+def test_edge_cases(gpu):
+    # Duplicate sources, an isolated source, a source with a self-loop-free
+    # single edge, and a graph with no edges.
+    h = cgp_amd.HostCSR.from_coo(6, [0, 1, 1, 2, 4], [1, 0, 2, 1, 4], [3, 3, 4, 4, 9])
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    kinds = [BFS, BFS, SSSP, SSSP, BFS, SSSP, WCC, BFS]
+    srcs = [0, 0, 1, 1, 5, 5, 0, 4]
+    out = cgp_amd.Engine(g).run_batch(kinds, srcs)[0]
+    assert np.array_equal(out, oracle_rows(og, kinds, srcs))
+    g0 = cgp_amd.Graph.from_csr([0, 0, 0, 0], [])
+    out = cgp_amd.Engine(g0).run_batch([BFS, SSSP, WCC], [2, 0, 0])[0]
+    assert out.tolist() == [[INF, INF, 0], [0, INF, INF], [0, 1, 2]]
+    en = cgp_amd.Engine(g)
+    with pytest.raises(cgp_amd.CgpError) as e:
+        en.run_batch([BFS], [6])
+    assert e.value.status == cgp_amd.CGP_EINVAL
+    with pytest.raises(cgp_amd.CgpError) as e:
+        en.run_batch([], [])
+    assert e.value.status == cgp_amd.CGP_EINVAL
+    with pytest.raises(cgp_amd.CgpError):
+        en.run_batch([7], [0])
+
+
+def test_directed_graph_uses_push_and_weak_components(gpu):
+    h = cgp_amd.HostCSR.from_coo(7, [0, 0, 2, 3, 5], [1, 2, 3, 4, 6], [1, 2, 3, 4, 5])
+    g = cgp_amd.Graph.from_host(h)
+    assert not g.symmetric
+    og = O.G.of(h)
+    out = cgp_amd.Engine(g, traversal_mode=cgp_amd.PULL).run_batch([BFS, SSSP, WCC], [0, 2, 0])[0]
+    assert np.array_equal(out[0], O.bfs(og, 0))
+    assert np.array_equal(out[1], O.sssp(og, 2))
+    assert out[2].tolist() == [0, 0, 0, 0, 0, 5, 5]
+
+
+def test_config_rmat16_32_bfs_bit_exact(gpu):
+    # RMAT scale 16, ef 16, seed 1; 32 BFS sources uniform over degree >= 1.
+    h, g, og = rmat(16, 16, 1, 0, 0)
+    src = h.sample_sources(32, seed=1)
+    out = cgp_amd.Engine(g).run_batch([BFS] * 32, src)[0]
+    assert np.array_equal(out, oracle_rows(og, [0] * 32, src))
+
+
+def test_device_output_and_reached_edges(gpu):
+    h, g, og = rmat(12, 8, 12, 1, 40)
+    src = h.sample_sources(20, seed=4)
+    kinds = [BFS] * 10 + [SSSP] * 8 + [WCC] * 2
+    srcs = list(src[:18]) + [0, 0]
+    buf = cgp_amd.DeviceBuffer(20 * g.V * 4)
+    _, st = cgp_amd.Engine(g).run_batch(kinds, srcs, out=buf)
+    out = buf.to_host(shape=(20, g.V))
+    assert np.array_equal(out, oracle_rows(og, kinds, srcs))
+    m = cgp_amd.reached_edges(g, buf, 20)
+    exp = [O.reached_edges(og, r) for r in out]
+    assert m.tolist() == exp
+
+
+def test_properties_at_scale_rmat20_sssp(gpu):
+    # RMAT scale 20, w in [1,64], 64 SSSP: every row passes the exact
+    # shortest-path characterisation; a sample of rows equals Dijkstra.
+    h, g, og = rmat(20, 16, 2, 1, 64)
+    src = h.sample_sources(64, seed=2)
+    say("rmat20 sssp start")
+    out = cgp_amd.Engine(g).run_batch([SSSP] * 64, src)[0]
+    for i in range(0, 64, 9):
+        assert check_sssp(og.off, og.idx, og.wt, int(src[i]), out[i]) is None
+    for i in (0, 63):
+        assert np.array_equal(out[i], O.sssp(og, src[i]))
