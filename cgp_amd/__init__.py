@@ -198,6 +198,12 @@ class DeviceBuffer:
         check(lib.cgp_copy_to_host(a.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
         return a.reshape(shape) if shape is not None else a
 
+    def view(self, offset, nbytes):
+        """A borrowed window [offset, offset+nbytes) of this buffer."""
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view out of range")
+        return _View(self, offset, nbytes)
+
     def close(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             lib.cgp_device_free(self.ptr)
@@ -205,6 +211,13 @@ class DeviceBuffer:
 
     def __del__(self):
         self.close()
+
+
+class _View:
+    def __init__(self, base, offset, nbytes):
+        self.base = base
+        self.ptr = C.c_void_p(base.ptr.value + offset)
+        self.nbytes = nbytes
 
 
 def default_opts(**kw):

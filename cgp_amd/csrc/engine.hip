@@ -112,11 +112,23 @@ __device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
 }
 
 // ------------------------------------------------------------------ push
+// Two frontier modes, chosen per iteration by the filter:
+//  sparse  the advance appends each newly touched vertex to a touched list
+//          (first touch detected with a returning atomicOr on next[] plus a
+//          per-iteration mark), and the filter visits only those vertices;
+//  dense   the advance issues fire-and-forget atomics only and the filter
+//          scans every vertex's next words.
+// SSSP improvements are detected optimistically in both: when the candidate
+// beats the value just read, the (non-returning) atomicMin is issued and the
+// query's next bit set.  A concurrent smaller write can only make that bit
+// redundant (the vertex re-relaxes with its true value), never lose an
+// improvement, because values only decrease.
+
 // Relaxes edge (u -> v, weight wv) for every active BFS bit (bitmask
-// first-write against visited) and every sparse SSSP bit (atomicMin on the
-// value); sets the improved bits in next[v] and reports whether this lane
-// made the vertex's first claim of the iteration.
-template <int W>
+// first-write against visited) and every sparse SSSP bit; sets the new bits
+// in next[v].  Sparse mode returns whether this lane made the vertex's first
+// claim of the iteration.
+template <int W, bool SPARSE>
 __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u32 wv,
                                            const u64 (&bm)[W], const u64 (&sm)[W], u32 stamp,
                                            int rshift, u64& nbfs, u64& nsssp, u64& nnew,
@@ -125,7 +137,10 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
   const size_t vb = (size_t)v * W;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
+    if (!(bm[w] | sm[w])) continue;
+    u64* pn = &P.next[vb + w];
     u64 add = 0;
+    const u64 nx = *pn;
     if (bm[w]) {
       ++nbfs;
       add = bm[w] & ~P.visited[vb + w];
@@ -142,24 +157,22 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
         if (cand < du) continue;  // saturate: never wrap past CGP_INF
         u32* pv = &P.vals[vr + q];
         if (cand < *pv) {
-          const u32 old = atomicMin(pv, cand);
-          if (cand < old) {
-            add |= 1ull << q;
-            ++nimp;
-          }
+          atomicMin(pv, cand);
+          add |= 1ull << q;
+          ++nimp;
         }
       }
     }
-    if (add) {
-      u64* pn = &P.next[vb + w];
-      if (add & ~*pn) {
-        const u64 old = atomicOr(pn, add);
-        if (add & ~old) ++nnew;
-        if (old == 0) first = true;
+    if (add & ~nx) {
+      ++nnew;
+      if (SPARSE) {
+        if (atomicOr(pn, add) == 0) first = true;
+      } else {
+        atomicOr(pn, add);
       }
     }
   }
-  return first && claim_vertex(P.mark, v, stamp);
+  return SPARSE && first && claim_vertex(P.mark, v, stamp);
 }
 
 template <int W>
@@ -172,280 +185,260 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, 
     const u64 f = P.front[(size_t)u * W + w];
     bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
     u64 s = f & cfg->val_bits[w];
-    if (__popcll(s) >= dense_t) s = 0;  // handled by the lane-per-query kernel
+    if (__popcll(s) >= dense_t) s = 0;  // handled lane-per-query
     sm[w] = s;
     any_w |= (s != 0);
   }
 }
 
-// G lanes per frontier vertex (G = 8 for the small bin, 64 for the medium bin).
-template <int W, int G>
-__global__ __launch_bounds__(BLOCK) void k_push_group(KP P, int which) {
-  Ctl* C = P.ctl;
-  if (C->finished) return;
-  const u32 iter = C->iter, p = iter & 1u;
-  const u32 n = C->bcount[p][which];
-  if (n == 0) return;
-  const int32_t* list = which == 0 ? P.small[p] : P.medium[p];
-  const bool bfs_on = C->bfs_pull == 0;
-  const u32 stamp = C->stamp_base + iter + 1;
-  const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
-  const int gl = (int)(threadIdx.x % G);
-  const u32 gid = (blockIdx.x * BLOCK + threadIdx.x) / G;
-  const u32 ngroups = (gridDim.x * BLOCK) / G;
-  u64 nitems = 0, nedges = 0, nwedges = 0, nbfs = 0, nsssp = 0, nnew = 0, nimp = 0;
-  for (u32 i = gid; i < n; i += ngroups) {
-    const int32_t u = list[i];
-    u64 bm[W], sm[W];
-    bool any_s;
-    load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
-    bool any_b = false;
+// Work counters of one lane, flushed once per kernel in profile mode.
+struct Tally {
+  u64 items = 0, edges = 0, wedges = 0, bfs = 0, sssp = 0, nnew = 0, imp = 0;
+  u64 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
+};
+
+// Expands frontier vertex u over edges [e0, e1) with lanes striding by
+// `stride` from `first` (lane-per-edge): BFS bits and sparse SSSP words.
+template <int W, bool SPARSE>
+__device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0, int32_t e1,
+                                             int32_t first, int32_t stride, bool bfs_on, int dense_t,
+                                             int rshift, u32 stamp, Tally& t) {
+  u64 bm[W], sm[W];
+  bool any_s;
+  load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+  bool any_b = false;
 #pragma unroll
-    for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
-    if (!any_b && !any_s) continue;
-    if (gl == 0) ++nitems;
-    const int32_t e0 = P.off[u], e1 = P.off[u + 1];
-    for (int32_t e = e0 + gl; e < e1; e += G) {
-      const int32_t v = P.idx[e];
-      u32 wv = 1;
-      if (any_s && P.wt) {
-        wv = P.wt[e];
-        ++nwedges;
-      }
-      ++nedges;
-      const bool claimed = relax_edge<W>(P, u, v, wv, bm, sm, stamp, rshift, nbfs, nsssp, nnew, nimp);
-      wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+  for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
+  if (!any_b && !any_s) return;
+  if (first == e0) ++t.items;
+  for (int32_t e = first; e < e1; e += stride) {
+    const int32_t v = P.idx[e];
+    u32 wv = 1;
+    if (any_s && P.wt) {
+      wv = P.wt[e];
+      ++t.wedges;
     }
-  }
-  if (P.profile) {
-    prof_add(P, PF_PUSH_ITEMS, nitems);
-    prof_add(P, PF_PUSH_EDGES, nedges);
-    prof_add(P, PF_PUSH_WEDGES, nwedges);
-    prof_add(P, PF_BFS_WORDS, nbfs);
-    prof_add(P, PF_SSSP_PAIRS, nsssp);
-    prof_add(P, PF_NEW_WORDS, nnew);
-    prof_add(P, PF_IMPROVED, nimp);
+    ++t.edges;
+    const bool claimed =
+        relax_edge<W, SPARSE>(P, u, v, wv, bm, sm, stamp, rshift, t.bfs, t.sssp, t.nnew, t.imp);
+    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
   }
 }
 
-// One workgroup per LCHUNK-edge chunk of a high-degree frontier vertex.
-template <int W>
-__global__ __launch_bounds__(BLOCK) void k_push_large(KP P) {
+// Dense SSSP word w of vertex u over edges [e0, e1): one wave, lane q is
+// query q of the word, edges walked in order with 8 value rows in flight per
+// step; each value access is one coalesced row.  Every lane of the wave calls it.
+template <int W, bool SPARSE>
+__device__ __forceinline__ void expand_dense(const KP& P, int32_t u, int w, int32_t e0, int32_t e1,
+                                             int rshift, u32 stamp, Tally& t) {
+  constexpr int U = 8;
+  const int lane = lane_id();
+  const int R = 1 << rshift;
+  const u64 sm = P.front[(size_t)u * W + w] & P.cfg->val_bits[w];
+  const bool act = lane < R && ((sm >> lane) & 1ull);
+  const u32 du = act ? P.vals[vrow(P.V, w, u, rshift) + lane] : INF;
+  if (lane == 0) {
+    ++t.ditems;
+    t.dedges += (u64)(e1 - e0);
+    t.sssp += (u64)(e1 - e0) * (u64)__popcll(sm);
+  }
+  for (int32_t eb = e0; eb < e1; eb += 64) {
+    const int cnt = min(64, e1 - eb);
+    const int32_t myv = lane < cnt ? P.idx[eb + lane] : 0;
+    const u32 myw = lane < cnt ? (P.wt ? P.wt[eb + lane] : 1u) : 0u;
+    for (int j = 0; j < cnt; j += U) {
+      int32_t vv[U];
+      u32 cand[U], cur[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int jj = min(j + k, cnt - 1);
+        vv[k] = __builtin_amdgcn_readlane(myv, jj);
+        const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, jj);
+        cand[k] = du + wk;
+        if (cand[k] < du || j + k >= cnt) cand[k] = INF;
+        cur[k] = (act && j + k < cnt) ? P.vals[vrow(P.V, w, vv[k], rshift) + lane] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const bool imp = act && cand[k] < cur[k];
+        if (imp) atomicMin(&P.vals[vrow(P.V, w, vv[k], rshift) + lane], cand[k]);
+        const u64 m = __ballot(imp);
+        if (m) {
+          t.imp += imp ? 1 : 0;
+          bool claimed = false;
+          if (lane == 0) {
+            ++t.nnew;
+            if (SPARSE) {
+              if (atomicOr(&P.next[(size_t)vv[k] * W + w], m) == 0)
+                claimed = claim_vertex(P.mark, vv[k], stamp);
+            } else {
+              atomicOr(&P.next[(size_t)vv[k] * W + w], m);
+            }
+          }
+          if (SPARSE) wave_append<int32_t>(claimed, vv[k], &P.ctl->tcount, P.touched);
+        }
+      }
+    }
+  }
+}
+
+// Bottom-up BFS step for vertex v (one lane) on a symmetric graph: an
+// unvisited (v, q) becomes visited at depth L+1 iff some neighbour u has q
+// visited (all visited bits at this point have depth <= L, and a neighbour
+// at depth < L would already have visited v).  visited is only written by
+// the filter, so the scan reads one consistent level.  Early exit once every
+// missing bit was found.
+template <int W, bool SPARSE>
+__device__ __forceinline__ bool pull_vertex(const KP& P, int32_t v, const u64 (&bb)[W], u32 stamp,
+                                            Tally& t) {
+  u64 tgt[W], acc[W];
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    tgt[w] = bb[w] & ~P.visited[(size_t)v * W + w];
+    acc[w] = 0;
+    any |= tgt[w] != 0;
+  }
+  if (!any) return false;
+  ++t.pverts;
+  const int32_t e1 = P.off[v + 1];
+  for (int32_t e = P.off[v]; e < e1; ++e) {
+    const int32_t u = P.idx[e];
+    ++t.pedges;
+    bool done = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (tgt[w] & ~acc[w]) {
+        ++t.pwords;
+        acc[w] |= P.visited[(size_t)u * W + w] & tgt[w];
+        done &= (acc[w] == tgt[w]);
+      }
+    }
+    if (done) break;
+  }
+  bool first = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (acc[w]) {
+      ++t.nnew;
+      if (SPARSE) {
+        if (atomicOr(&P.next[(size_t)v * W + w], acc[w]) == 0) first = true;
+      } else {
+        atomicOr(&P.next[(size_t)v * W + w], acc[w]);
+      }
+    }
+  }
+  return SPARSE && first && claim_vertex(P.mark, v, stamp);
+}
+
+__device__ __forceinline__ u32 wave_fetch(u32* ctr, u32 n) {
+  u32 i = 0;
+  if (lane_id() == 0) i = atomicAdd(ctr, n);
+  return (u32)__shfl((int)i, 0, 64);
+}
+
+// Body of the advance for one frontier mode; workgroups pull work
+// dynamically, heaviest first:
+//   L  one workgroup per LCHUNK-edge chunk of a high-degree vertex
+//   D  one wave per (vertex, dense SSSP word, DCHUNK-edge chunk), lane = query
+//   M  one wave per medium-degree vertex, lane = edge
+//   S  8 lanes per low-degree vertex, lane = edge
+//   P  (pull iterations) one lane per vertex of the whole graph
+template <int W, bool SPARSE>
+__device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t) {
   Ctl* C = P.ctl;
-  if (C->finished) return;
   const u32 iter = C->iter, p = iter & 1u;
-  const u32 n = C->bcount[p][2];
-  if (n == 0) return;
   const bool bfs_on = C->bfs_pull == 0;
   const u32 stamp = C->stamp_base + iter + 1;
   const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
-  u64 nitems = 0, nedges = 0, nwedges = 0, nbfs = 0, nsssp = 0, nnew = 0, nimp = 0;
-  for (u32 i = blockIdx.x; i < n; i += gridDim.x) {
+  const int lane = lane_id();
+  // Phase L.
+  const u32 nL = C->bcount[p][2];
+  while (nL) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_item = atomicAdd(&C->fetch[p][0], 1u);
+    __syncthreads();
+    const u32 i = *s_item;
+    if (i >= nL) break;
     const u64 item = P.large[p][i];
     const int32_t u = (int32_t)(item >> 32);
-    const u32 chunk = (u32)item;
-    u64 bm[W], sm[W];
-    bool any_s;
-    load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
-    bool any_b = false;
-#pragma unroll
-    for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
-    if (!any_b && !any_s) continue;
-    if (threadIdx.x == 0) ++nitems;
-    const int32_t e0 = P.off[u] + (int32_t)(chunk * LCHUNK);
+    const int32_t e0 = P.off[u] + (int32_t)((u32)item * LCHUNK);
     const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
-    for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += BLOCK) {
-      const int32_t v = P.idx[e];
-      u32 wv = 1;
-      if (any_s && P.wt) {
-        wv = P.wt[e];
-        ++nwedges;
-      }
-      ++nedges;
-      const bool claimed = relax_edge<W>(P, u, v, wv, bm, sm, stamp, rshift, nbfs, nsssp, nnew, nimp);
-      wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
-    }
+    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BLOCK, bfs_on, dense_t, rshift,
+                            stamp, t);
   }
-  if (P.profile) {
-    prof_add(P, PF_PUSH_ITEMS, nitems);
-    prof_add(P, PF_PUSH_EDGES, nedges);
-    prof_add(P, PF_PUSH_WEDGES, nwedges);
-    prof_add(P, PF_BFS_WORDS, nbfs);
-    prof_add(P, PF_SSSP_PAIRS, nsssp);
-    prof_add(P, PF_NEW_WORDS, nnew);
-    prof_add(P, PF_IMPROVED, nimp);
-  }
-}
-
-// Dense SSSP words: one wave per (vertex, word, DCHUNK-edge chunk), lane q is
-// query q of the word, the wave walks the edges in order and each lane's
-// value access is one coalesced 256 B row per edge.
-template <int W>
-__global__ __launch_bounds__(BLOCK) void k_push_dense(KP P) {
-  Ctl* C = P.ctl;
-  if (C->finished) return;
-  const u32 iter = C->iter, p = iter & 1u;
-  const u32 n = C->bcount[p][3];
-  if (n == 0) return;
-  const u32 stamp = C->stamp_base + iter + 1;
-  const int rshift = P.cfg->rshift;
-  const int R = 1 << rshift;
-  const int lane = lane_id();
-  const u32 wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6;
-  const u32 nwaves = (gridDim.x * BLOCK) >> 6;
-  u64 nitems = 0, nedges = 0, nsssp = 0, nnew = 0, nimp = 0;
-  for (u32 i = wave; i < n; i += nwaves) {
+  // Phase D.
+  const u32 nD = C->bcount[p][3];
+  while (nD) {
+    const u32 i = wave_fetch(&C->fetch[p][1], 1u);
+    if (i >= nD) break;
     const u64 item = P.dense[p][i];
     const int32_t u = (int32_t)(item >> 32);
     const int w = (int)((item >> 24) & 0xFF);
-    const u32 chunk = (u32)(item & 0xFFFFFF);
-    const u64 sm = P.front[(size_t)u * W + w] & P.cfg->val_bits[w];
-    const int32_t e0 = P.off[u] + (int32_t)(chunk * DCHUNK);
+    const int32_t e0 = P.off[u] + (int32_t)((u32)(item & 0xFFFFFF) * DCHUNK);
     const int32_t e1 = min(P.off[u + 1], e0 + DCHUNK);
-    const bool act = lane < R && ((sm >> lane) & 1ull);
-    const u32 du = act ? P.vals[vrow(P.V, w, u, rshift) + lane] : INF;
-    if (lane == 0) {
-      ++nitems;
-      nedges += (u64)(e1 - e0);
-      nsssp += (u64)(e1 - e0) * (u64)__popcll(sm);
-    }
-    for (int32_t eb = e0; eb < e1; eb += 64) {
-      const int cnt = min(64, e1 - eb);
-      const int32_t myv = lane < cnt ? P.idx[eb + lane] : 0;
-      const u32 myw = lane < cnt ? (P.wt ? P.wt[eb + lane] : 1u) : 0u;
-      int j = 0;
-      for (; j + 4 <= cnt; j += 4) {
-        int32_t vv[4];
-        u32 cand[4], cur[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          vv[k] = __builtin_amdgcn_readlane(myv, j + k);
-          const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, j + k);
-          cand[k] = du + wk;
-          if (cand[k] < du) cand[k] = INF;
-          cur[k] = act ? P.vals[vrow(P.V, w, vv[k], rshift) + lane] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          bool imp = false;
-          if (act && cand[k] < cur[k]) {
-            const u32 old = atomicMin(&P.vals[vrow(P.V, w, vv[k], rshift) + lane], cand[k]);
-            imp = cand[k] < old;
-          }
-          const u64 m = __ballot(imp);
-          if (m) {
-            nimp += imp ? 1 : 0;
-            bool claimed = false;
-            if (lane == 0) {
-              const u64 old = atomicOr(&P.next[(size_t)vv[k] * W + w], m);
-              if (m & ~old) ++nnew;
-              if (old == 0) claimed = claim_vertex(P.mark, vv[k], stamp);
-            }
-            wave_append<int32_t>(claimed, vv[k], &C->tcount, P.touched);
-          }
-        }
-      }
-      for (; j < cnt; ++j) {
-        const int32_t v = __builtin_amdgcn_readlane(myv, j);
-        const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, j);
-        u32 cand = du + wk;
-        if (cand < du) cand = INF;
-        bool imp = false;
-        if (act) {
-          u32* pv = &P.vals[vrow(P.V, w, v, rshift) + lane];
-          if (cand < *pv) {
-            const u32 old = atomicMin(pv, cand);
-            imp = cand < old;
-          }
-        }
-        const u64 m = __ballot(imp);
-        if (m) {
-          nimp += imp ? 1 : 0;
-          bool claimed = false;
-          if (lane == 0) {
-            const u64 old = atomicOr(&P.next[(size_t)v * W + w], m);
-            if (m & ~old) ++nnew;
-            if (old == 0) claimed = claim_vertex(P.mark, v, stamp);
-          }
-          wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
-        }
-      }
+    expand_dense<W, SPARSE>(P, u, w, e0, e1, rshift, stamp, t);
+  }
+  // Phase M.
+  const u32 nM = C->bcount[p][1];
+  while (nM) {
+    const u32 i = wave_fetch(&C->fetch[p][2], 1u);
+    if (i >= nM) break;
+    const int32_t u = P.medium[p][i];
+    const int32_t e0 = P.off[u], e1 = P.off[u + 1];
+    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, t);
+  }
+  // Phase S.
+  const u32 nS = C->bcount[p][0];
+  while (nS) {
+    const u32 base = wave_fetch(&C->fetch[p][3], 8u);
+    if (base >= nS) break;
+    const u32 i = base + (u32)(lane >> 3);
+    if (i < nS) {
+      const int32_t u = P.small[p][i];
+      const int32_t e0 = P.off[u], e1 = P.off[u + 1];
+      expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (lane & 7), 8, bfs_on, dense_t, rshift, stamp, t);
     }
   }
-  if (P.profile) {
-    prof_add(P, PF_DENSE_ITEMS, nitems);
-    prof_add(P, PF_DENSE_EDGES, nedges);
-    prof_add(P, PF_SSSP_PAIRS, nsssp);
-    prof_add(P, PF_NEW_WORDS, nnew);
-    prof_add(P, PF_IMPROVED, nimp);
+  // Phase P.
+  if (!bfs_on) {
+    u64 bb[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < P.V; base += (int64_t)gridDim.x * BLOCK) {
+      const int32_t v = (int32_t)(base + threadIdx.x);
+      const bool claimed = v < P.V && pull_vertex<W, SPARSE>(P, v, bb, stamp, t);
+      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+    }
   }
 }
 
-// ------------------------------------------------------------------ pull
-// Bottom-up BFS step on a symmetric graph: an unvisited (v, q) becomes
-// visited at depth L+1 iff some neighbour u has q visited (all visited bits
-// at this point have depth <= L, and a neighbour at depth < L would already
-// have visited v).  visited is not written until the filter runs, so the
-// scan reads a consistent level.  One lane per vertex with early exit once
-// every missing bit was found.
+// The advance of one iteration, as one persistent launch.
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_pull(KP P) {
+__global__ __launch_bounds__(BLOCK) void k_advance(KP P) {
   Ctl* C = P.ctl;
-  if (C->finished || C->bfs_pull == 0) return;
-  const u32 iter = C->iter;
-  const u32 stamp = C->stamp_base + iter + 1;
-  u64 bb[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
-  u64 nverts = 0, nwords = 0, nedges = 0, nnew = 0;
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < P.V; base += (int64_t)gridDim.x * BLOCK) {
-    const int32_t v = (int32_t)(base + threadIdx.x);
-    const bool valid = v < P.V;
-    u64 tgt[W], acc[W];
-    bool any = false;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      tgt[w] = valid ? (bb[w] & ~P.visited[(size_t)v * W + w]) : 0ull;
-      acc[w] = 0;
-      any |= tgt[w] != 0;
-    }
-    bool claimed = false;
-    if (any) {
-      ++nverts;
-      const int32_t e1 = P.off[v + 1];
-      for (int32_t e = P.off[v]; e < e1; ++e) {
-        const int32_t u = P.idx[e];
-        ++nedges;
-        bool done = true;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          if (tgt[w] & ~acc[w]) {
-            ++nwords;
-            acc[w] |= P.visited[(size_t)u * W + w] & tgt[w];
-            done &= (acc[w] == tgt[w]);
-          }
-        }
-        if (done) break;
-      }
-      bool first = false;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        if (acc[w]) {
-          const u64 old = atomicOr(&P.next[(size_t)v * W + w], acc[w]);
-          ++nnew;
-          if (old == 0) first = true;
-        }
-      }
-      claimed = first && claim_vertex(P.mark, v, stamp);
-    }
-    wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
-  }
+  if (C->finished) return;
+  __shared__ u32 s_item;
+  if (threadIdx.x == 0) atomicMin(&C->adv_t0, (u64)wall_clock64());
+  Tally t;
+  if (C->dense_mode) advance_body<W, false>(P, &s_item, t);
+  else advance_body<W, true>(P, &s_item, t);
   if (P.profile) {
-    prof_add(P, PF_PULL_VERTS, nverts);
-    prof_add(P, PF_PULL_WORDS, nwords);
-    prof_add(P, PF_PULL_EDGES, nedges);
-    prof_add(P, PF_NEW_WORDS, nnew);
+    prof_add(P, PF_PUSH_ITEMS, t.items);
+    prof_add(P, PF_PUSH_EDGES, t.edges);
+    prof_add(P, PF_PUSH_WEDGES, t.wedges);
+    prof_add(P, PF_BFS_WORDS, t.bfs);
+    prof_add(P, PF_SSSP_PAIRS, t.sssp);
+    prof_add(P, PF_NEW_WORDS, t.nnew);
+    prof_add(P, PF_IMPROVED, t.imp);
+    prof_add(P, PF_DENSE_ITEMS, t.ditems);
+    prof_add(P, PF_DENSE_EDGES, t.dedges);
+    prof_add(P, PF_PULL_VERTS, t.pverts);
+    prof_add(P, PF_PULL_WORDS, t.pwords);
+    prof_add(P, PF_PULL_EDGES, t.pedges);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(&C->adv_t1, (u64)wall_clock64());
 }
 
 // ------------------------------------------------------------------ seed
@@ -470,19 +463,22 @@ __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, i
 }
 
 // ------------------------------------------------------------------ filter
-// Claims the next words of every touched vertex, settles BFS bits (visited
-// |= new, depth = iteration+1 - slot start, written as coalesced rows),
-// publishes the frontier words, and emits the degree-binned next frontier
-// with wave-level prefix sums.  The last workgroup to finish advances the
-// iteration, resets the consumed counters, and picks push or pull.
+// Claims the next words of every touched vertex (sparse mode: the touched
+// list; dense mode: every vertex), settles BFS bits (visited |= new, depth =
+// iteration+1 - slot start, written as coalesced rows), publishes the
+// frontier words, and emits the degree-binned next frontier with wave-level
+// prefix sums.  The last workgroup to finish advances the iteration, resets
+// the consumed counters, and picks push or pull and the next frontier mode.
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   Ctl* C = P.ctl;
   if (C->finished) return;
   __shared__ int s_last;
+  if (threadIdx.x == 0) atomicMin(&C->flt_t0, (u64)wall_clock64());
   const Cfg* cfg = P.cfg;
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
-  const u32 n = C->tcount;
+  const bool scan = C->dense_mode != 0;
+  const u32 n = scan ? (u32)P.V : C->tcount;
   const int rshift = cfg->rshift, dense_t = cfg->dense_t;
   const int lane = lane_id();
   u64 bb[W], vb[W], un[W];
@@ -493,22 +489,27 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     un[w] = 0;
   }
   const u32 depth_base = iter + 1u;
-  u64 pairs = 0, epairs = 0;
+  u64 pairs = 0, epairs = 0, fedges = 0;
   for (u32 base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
     const u32 i = base + threadIdx.x;
     const bool valid = i < n;
-    const int32_t v = valid ? P.touched[i] : 0;
-    const int32_t deg = valid ? P.off[v + 1] - P.off[v] : 0;
+    const int32_t v = valid ? (scan ? (int32_t)i : P.touched[i]) : 0;
+    u64 nwv[W];
+    bool anyw = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      nwv[w] = valid ? P.next[(size_t)v * W + w] : 0ull;
+      anyw |= nwv[w] != 0;
+    }
+    const int32_t deg = anyw ? P.off[v + 1] - P.off[v] : 0;
     u64 nb[W];
     bool need_bin = false;
     u32 dmask = 0, vpairs = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      u64 nw = 0;
-      if (valid) {
-        u64* pn = &P.next[(size_t)v * W + w];
-        nw = *pn;
-        if (nw) *pn = 0;
+      const u64 nw = nwv[w];
+      if (anyw) {
+        if (nw) P.next[(size_t)v * W + w] = 0;
         P.front[(size_t)v * W + w] = nw;
       }
       nb[w] = nw & bb[w];
@@ -522,6 +523,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     }
     pairs += vpairs;
     epairs += (u64)vpairs * (u64)deg;
+    fedges += (u64)deg;
     if (deg == 0) {
       need_bin = false;
       dmask = 0;
@@ -569,9 +571,11 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   // Block-level reductions of the frontier statistics.
   pairs = wave_sum64(pairs);
   epairs = wave_sum64(epairs);
+  fedges = wave_sum64(fedges);
   if (lane == 0) {
     if (pairs) atomicAdd(&C->bfs_pairs[np], pairs);
     if (epairs) atomicAdd(&C->bfs_edges[np], epairs);
+    if (fedges) atomicAdd(&C->fedges[np], fedges);
   }
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -591,11 +595,15 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     const u32 total = atomicAdd(&C->bcount[np][0], 0u) + atomicAdd(&C->bcount[np][1], 0u) +
                       atomicAdd(&C->bcount[np][2], 0u) + atomicAdd(&C->bcount[np][3], 0u);
     const u64 fp = atomicAdd(&C->bfs_pairs[np], 0ull), fe = atomicAdd(&C->bfs_edges[np], 0ull);
+    const u64 fall = atomicAdd(&C->fedges[np], 0ull);
+    C->dense_mode = fall >= cfg->dense_fedges_thr ? 1u : 0u;
+    C->fedges[p] = 0;
     for (int w = 0; w < MAXW; ++w) {
       C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
     }
     C->tcount = 0;
     C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = 0;
+    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = 0;
     C->bfs_pairs[p] = 0;
     C->bfs_edges[p] = 0;
     C->done_blocks = 0;
@@ -608,6 +616,15 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       }
     }
     C->bfs_pull = pull;
+    // Launch timing: the advance of this iteration ran iff iter != seeding.
+    if (iter != 0xFFFFFFFFu) {
+      C->adv_ticks += C->adv_t1 - C->adv_t0;
+      C->adv_launches += 1;
+      C->flt_ticks += (u64)wall_clock64() - C->flt_t0;
+    }
+    C->adv_t0 = ~0ull;
+    C->adv_t1 = 0;
+    C->flt_t0 = ~0ull;
     const u32 next_iter = iter + 1u;
     C->iter = next_iter;
     if (total == 0 || (C->max_iter != 0 && next_iter >= C->max_iter)) {
@@ -615,6 +632,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     } else {
       if (pull) ++C->pull_iters;
       else ++C->push_iters;
+      if (C->dense_mode) ++C->dense_iters;
     }
     __threadfence();
   }
@@ -803,7 +821,7 @@ struct cgp_engine {
   int cus = 1;
   int wcap = 0;            // words allocated in visited/next/front/dense
   DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
-  DevBuf parent, wcc_rows, outbuf, qmap, slot_src, prof_out;
+  DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
   Ctl* d_ctl = nullptr;
   Cfg* d_cfg = nullptr;
   Ctl* h_ctl = nullptr;     // pinned
@@ -814,12 +832,20 @@ struct cgp_engine {
   bool next_dirty = false;
   bool busy = false;
   std::mutex mu;
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
-  int gexec_iters[4] = {0, 0, 0, 0};
-  // Per-launch timing (profile mode).
+  // Captured iteration graphs, one per mask-word width.
+  struct GExec {
+    hipGraphExec_t exec = nullptr;
+    int K = 0;
+    int prof = -1;
+  };
+  GExec gx[4];
+  double clock_khz = 100000.0;  // device wall-clock rate (wall_clock64 ticks)
+  // Per-phase timing of init / WCC / output (launch timing on, outside graphs).
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
+  bool timed() const { return (opts.profile & 1) != 0; }
+  bool counted() const { return (opts.profile & 2) != 0; }
   // Serving session state lives in serve.hip (attached here).
   void* serve = nullptr;
 
@@ -827,7 +853,7 @@ struct cgp_engine {
     KP P{};
     P.V = g->V;
     P.E = g->E;
-    P.profile = opts.profile;
+    P.profile = (opts.profile & 2) ? 1 : 0;
     P.off = g->off;
     P.idx = g->idx;
     P.wt = g->weighted ? g->wt : nullptr;
@@ -853,21 +879,16 @@ struct cgp_engine {
 
   void drop_graphs() {
     for (int k = 0; k < 4; ++k) {
-      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
-      gexec[k] = nullptr;
-      gexec_iters[k] = 0;
+      if (gx[k].exec) (void)hipGraphExecDestroy(gx[k].exec);
+      gx[k].exec = nullptr;
+      gx[k].K = 0;
+      gx[k].prof = -1;
     }
   }
 
   // Grid sizes: persistent-style, sized to the device, independent of the
   // frontier so the launches can be captured once and replayed.
-  int grid_group() const { return cus * 8; }
-  int grid_large() const { return cus * 4; }
-  int grid_dense() const { return cus * 8; }
-  int grid_pull() const {
-    int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 16));
-  }
+  int grid_advance() const { return cus * 8; }
   int grid_filter() const {
     int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
     return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
@@ -909,9 +930,9 @@ struct cgp_engine {
     h_stage_cap = ints;
   }
 
-  // ---- timing helpers (profile mode only)
+  // ---- timing helpers (launch timing on)
   void tick(int cat, bool begin) {
-    if (!opts.profile) return;
+    if (!timed()) return;
     if (ev_used >= ev_pool.size()) {
       hipEvent_t e;
       HIP_CHECK(hipEventCreate(&e));
@@ -928,19 +949,12 @@ namespace {
 
 enum { CAT_INIT = 0, CAT_ADV = 1, CAT_FILTER = 2, CAT_WCC = 3, CAT_OUT = 4 };
 
+// One iteration: advance then filter (both time themselves on the device clock).
 template <int W>
-void launch_iteration(cgp_engine* en, const KP& P, bool with_pull) {
+void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
-  en->tick(CAT_ADV, true);
-  hipLaunchKernelGGL((k_push_large<W>), dim3(en->grid_large()), dim3(BLOCK), 0, s, P);
-  hipLaunchKernelGGL((k_push_group<W, 64>), dim3(en->grid_group()), dim3(BLOCK), 0, s, P, 1);
-  hipLaunchKernelGGL((k_push_group<W, 8>), dim3(en->grid_group()), dim3(BLOCK), 0, s, P, 0);
-  hipLaunchKernelGGL((k_push_dense<W>), dim3(en->grid_dense()), dim3(BLOCK), 0, s, P);
-  if (with_pull) hipLaunchKernelGGL((k_pull<W>), dim3(en->grid_pull()), dim3(BLOCK), 0, s, P);
-  en->tick(CAT_ADV, false);
-  en->tick(CAT_FILTER, true);
+  hipLaunchKernelGGL((k_advance<W>), dim3(en->grid_advance()), dim3(BLOCK), 0, s, P);
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
-  en->tick(CAT_FILTER, false);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -952,12 +966,12 @@ void launch_seed(cgp_engine* en, const KP& P, const int32_t* d_slot_src, int nsl
   HIP_CHECK(hipGetLastError());
 }
 
-void dispatch_iteration(cgp_engine* en, int W, const KP& P, bool with_pull) {
+void dispatch_iteration(cgp_engine* en, int W, const KP& P) {
   switch (W) {
-    case 1: launch_iteration<1>(en, P, with_pull); break;
-    case 2: launch_iteration<2>(en, P, with_pull); break;
-    case 4: launch_iteration<4>(en, P, with_pull); break;
-    default: launch_iteration<8>(en, P, with_pull); break;
+    case 1: launch_iteration<1>(en, P); break;
+    case 2: launch_iteration<2>(en, P); break;
+    case 4: launch_iteration<4>(en, P); break;
+    default: launch_iteration<8>(en, P); break;
   }
 }
 
@@ -979,38 +993,40 @@ void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
 
 int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
 
-// Runs the frontier iterations of one pass to convergence.
-void run_iterations(cgp_engine* en, int W, bool with_pull) {
+// Runs the frontier iterations of one pass to convergence: replays of a
+// captured K-iteration graph (or direct launches), one host readback of the
+// control block per replay.
+void run_iterations(cgp_engine* en, int W) {
   const KP P = en->kp();
-  const bool use_graph = en->opts.use_hip_graph && !en->opts.profile;
-  const int K = std::max(1, en->opts.graph_iterations);
-  if (use_graph) {
-    const int li = wlog(W);
-    if (!en->gexec[li] || en->gexec_iters[li] != K * (with_pull ? 1 : -1)) {
-      if (en->gexec[li]) (void)hipGraphExecDestroy(en->gexec[li]);
-      en->gexec[li] = nullptr;
-      hipGraph_t graph = nullptr;
-      HIP_CHECK(hipStreamBeginCapture(en->s, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < K; ++k) dispatch_iteration(en, W, P, with_pull);
-      HIP_CHECK(hipStreamEndCapture(en->s, &graph));
-      hipError_t ie = hipGraphInstantiate(&en->gexec[li], graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HIP_CHECK(ie);
-      en->gexec_iters[li] = K * (with_pull ? 1 : -1);
-    }
+  const bool use_graph = en->opts.use_hip_graph != 0;
+  const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
+  cgp_engine::GExec& gx = en->gx[wlog(W)];
+  if (use_graph && (!gx.exec || gx.K != K || gx.prof != en->opts.profile)) {
+    if (gx.exec) (void)hipGraphExecDestroy(gx.exec);
+    gx.exec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(en->s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; ++k) dispatch_iteration(en, W, P);
+    HIP_CHECK(hipStreamEndCapture(en->s, &graph));
+    hipError_t ie = hipGraphInstantiate(&gx.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(ie);
+    gx.K = K;
+    gx.prof = en->opts.profile;
   }
   for (;;) {
     if (use_graph) {
-      HIP_CHECK(hipGraphLaunch(en->gexec[wlog(W)], en->s));
+      HIP_CHECK(hipGraphLaunch(gx.exec, en->s));
     } else {
-      dispatch_iteration(en, W, P, with_pull);
+      dispatch_iteration(en, W, P);
     }
     HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
+    const u32 after = en->h_ctl->iter;
     if (en->h_ctl->finished) break;
     // Frontier iterations are bounded by V + 1 (each one settles at least one
     // more hop of some shortest path); more means a broken state machine.
-    if ((int64_t)en->h_ctl->iter > (int64_t)en->g->V + 2)
+    if ((int64_t)after > (int64_t)en->g->V + 2)
       cgpi::fail(CGP_EDEVICE, "traversal did not converge within V+2 iterations");
   }
 }
@@ -1020,10 +1036,26 @@ void run_iterations(cgp_engine* en, int W, bool with_pull) {
 // ------------------------------------------------------------------ batch run
 namespace cgpi {
 
-struct PassPlan {
-  std::vector<int32_t> qi;  // user query index per slot
-  int nbfs = 0, nsssp = 0;
-};
+// Algorithmic bytes of the advance launches of one pass (each byte at its
+// natural width, counted once, no cache-line waste):
+//   frontier item (vertex or chunk): 8 (offset pair) + 8 W (frontier words)
+//   edge scanned lane-per-edge: 4 (column index) [+4 weight when SSSP bits are active]
+//   (edge, active BFS word): 8 (visited word of the target)
+//   (edge, active SSSP query): 4 (target value) ; lowered value: 4 (store)
+//   next word gaining bits: 16 (read + write of the next-frontier word)
+//   dense item: 8 (offset pair) + 8 (frontier word); dense edge: 4 index + 4 weight
+//   pull: 8 + 8 W per vertex examined, 4 per edge, 8 per (edge, word)
+void add_work_bytes(cgp_run_stats& st, const u64* pf, int W) {
+  const double Wd = 8.0 * W;
+  st.advance_bytes += (double)pf[PF_PUSH_ITEMS] * (8.0 + Wd) + (double)pf[PF_PUSH_EDGES] * 4.0 +
+                      (double)pf[PF_PUSH_WEDGES] * 4.0 + (double)pf[PF_BFS_WORDS] * 8.0 +
+                      (double)pf[PF_SSSP_PAIRS] * 4.0 + (double)pf[PF_IMPROVED] * 4.0 +
+                      (double)pf[PF_NEW_WORDS] * 16.0 + (double)pf[PF_DENSE_ITEMS] * 16.0 +
+                      (double)pf[PF_DENSE_EDGES] * 8.0 + (double)pf[PF_PULL_VERTS] * (8.0 + Wd) +
+                      (double)pf[PF_PULL_EDGES] * 4.0 + (double)pf[PF_PULL_WORDS] * 8.0;
+  st.frontier_vertices += (int64_t)(pf[PF_PUSH_ITEMS] + pf[PF_DENSE_ITEMS]);
+  st.edges_scanned += (int64_t)(pf[PF_PUSH_EDGES] + pf[PF_DENSE_EDGES] + pf[PF_PULL_EDGES]);
+}
 
 void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
                       bool out_dev, cgp_run_stats* stats) {
@@ -1055,7 +1087,6 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->outbuf.ensure((size_t)n * V * 4);
     dout = (u32*)en->outbuf.p;
   }
-  if (en->opts.profile) HIP_CHECK(hipMemsetAsync(en->d_ctl, 0, sizeof(Ctl), en->s));
 
   // Passes of at most max_words*64 slots; BFS slots first, then SSSP.
   const int slots_max = std::max(1, std::min(en->opts.max_words, MAXW)) * 64;
@@ -1100,8 +1131,15 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
         cfg.val_bits[w] |= 1ull << q;
       }
     }
-    const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH;
+    // Bottom-up steps pay off only when few BFS queries share the words: with
+    // many, the missing bits of a vertex are rarely all found early.  Hybrid
+    // mode therefore pulls only for batches of at most kPullMaxQueries BFS
+    // queries; CGP_PULL forces it.
+    constexpr int kPullMaxQueries = 4;
+    const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH &&
+                          (en->opts.traversal_mode == CGP_PULL || nb <= kPullMaxQueries);
     cfg.pull_policy = !can_pull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
+    cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / 8);
     cfg.pull_pairs_thr = (u64)(en->opts.pull_frontier_ratio * (double)nb * (double)V);
     cfg.pull_edges_thr = (u64)(en->opts.pull_edge_ratio * (double)nb * (double)g->E);
     cfg.push_pairs_thr = cfg.pull_pairs_thr / 4 + 1;
@@ -1111,6 +1149,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     Ctl& c = *en->h_ctl;
     std::memset(&c, 0, sizeof(Ctl));
     c.iter = 0xFFFFFFFFu;
+    c.adv_t0 = ~0ull;
+    c.flt_t0 = ~0ull;
     if (en->stamp_epoch > 0xF0000000u) {
       HIP_CHECK(hipMemsetAsync(en->mark.p, 0, (size_t)V * 4, en->s));
       en->stamp_epoch = 1;
@@ -1120,17 +1160,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->slot_src.ensure((size_t)W * 64 * 4);
     en->qmap.ensure((size_t)W * 64 * 4);
     en->tick(CAT_INIT, true);
-    // Profile counters are accumulated over passes: keep them on the device.
-    if (en->opts.profile) {
-      en->prof_out.ensure(sizeof(u64) * NPROF);
-      HIP_CHECK(hipMemcpyAsync(en->prof_out.p, &en->d_ctl->prof[0], sizeof(u64) * NPROF,
-                               hipMemcpyDeviceToDevice, en->s));
-    }
     HIP_CHECK(hipMemcpyAsync(en->d_cfg, &cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, en->s));
-    if (en->opts.profile)
-      HIP_CHECK(hipMemcpyAsync(&en->d_ctl->prof[0], en->prof_out.p, sizeof(u64) * NPROF,
-                               hipMemcpyDeviceToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->slot_src.p, h_src, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->qmap.p, h_qmap, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
     fill_u32(en->s, en->vals.p, (size_t)W * V * R, INF, en->cus);
@@ -1141,12 +1172,18 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     }
     dispatch_seed(en, W, P, (const int32_t*)en->slot_src.p, W * 64);
     en->tick(CAT_INIT, false);
-    run_iterations(en, W, can_pull);
+    run_iterations(en, W);
     const Ctl& hc = *en->h_ctl;
     st.iterations += (int32_t)hc.iter;
     st.push_iterations += (int32_t)hc.push_iters;
     st.pull_iterations += (int32_t)hc.pull_iters;
     st.passes += 1;
+    if (en->counted()) add_work_bytes(st, hc.prof, W);
+    if (en->timed()) {
+      st.advance_ms += (double)hc.adv_ticks / en->clock_khz;
+      st.filter_ms += (double)hc.flt_ticks / en->clock_khz;
+      st.advance_launches += (int64_t)hc.adv_launches;
+    }
     en->stamp_epoch += hc.iter + 2;
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
     en->tick(CAT_OUT, true);
@@ -1179,37 +1216,16 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   if (!out_dev) {
     HIP_CHECK(hipMemcpy(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost));
   }
-  if (en->opts.profile) {
-    HIP_CHECK(hipMemcpy(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    const u64* pf = en->h_ctl->prof;
+  if (en->timed()) {
     for (auto& e : en->ev) {
       float ms = 0;
       if (e.second.second) HIP_CHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
       switch (e.first) {
         case CAT_INIT: st.init_ms += ms; break;
-        case CAT_ADV: st.advance_ms += ms; st.advance_launches += 1; break;
-        case CAT_FILTER: st.filter_ms += ms; break;
         case CAT_WCC: st.wcc_ms += ms; break;
         default: st.output_ms += ms; break;
       }
     }
-    // Algorithmic bytes of the advance launches (each byte at its natural
-    // width, counted once):
-    //   frontier item: 8 (offset pair) + 8 W (frontier words)
-    //   edge scanned:  4 (column index) [+4 weight when SSSP bits are active]
-    //   (edge, BFS word): 8 (visited word of the target)
-    //   (edge, SSSP query): 4 (target value read) ; improved value: 4 (store)
-    //   next word gaining bits: 16 (read + write of the next-frontier word)
-    //   pull: 8 + 8 W per vertex examined, 4 per edge, 8 per (edge, word)
-    const double Wd = 8.0 * std::max(1, en->wcap);
-    st.advance_bytes = (double)pf[PF_PUSH_ITEMS] * (8.0 + Wd) + (double)pf[PF_PUSH_EDGES] * 4.0 +
-                       (double)pf[PF_PUSH_WEDGES] * 4.0 + (double)pf[PF_BFS_WORDS] * 8.0 +
-                       (double)pf[PF_SSSP_PAIRS] * 4.0 + (double)pf[PF_IMPROVED] * 4.0 +
-                       (double)pf[PF_NEW_WORDS] * 16.0 + (double)pf[PF_DENSE_ITEMS] * 16.0 +
-                       (double)pf[PF_DENSE_EDGES] * 8.0 + (double)pf[PF_PULL_VERTS] * (8.0 + Wd) +
-                       (double)pf[PF_PULL_EDGES] * 4.0 + (double)pf[PF_PULL_WORDS] * 8.0;
-    st.frontier_vertices = (int64_t)(pf[PF_PUSH_ITEMS] + pf[PF_DENSE_ITEMS]);
-    st.edges_scanned = (int64_t)(pf[PF_PUSH_EDGES] + pf[PF_DENSE_EDGES] + pf[PF_PULL_EDGES]);
   }
   if (stats) *stats = st;
 }
@@ -1386,6 +1402,13 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.bin_large < en->opts.bin_small) en->opts.bin_large = en->opts.bin_small;
     en->opts.block_size = BLOCK;
     en->cus = device_cus();
+    {
+      int dev = 0;
+      int khz = 0;
+      HIP_CHECK(hipGetDevice(&dev));
+      if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+        en->clock_khz = (double)khz;
+    }
     HIP_CHECK(hipStreamCreateWithFlags(&en->s, hipStreamNonBlocking));
     HIP_CHECK(hipMalloc((void**)&en->d_ctl, sizeof(Ctl)));
     HIP_CHECK(hipMalloc((void**)&en->d_cfg, sizeof(Cfg)));
@@ -1403,7 +1426,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   DevBuf* bufs[] = {&en->visited, &en->next, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
                     &en->large[1], &en->dense[0], &en->dense[1], &en->parent, &en->wcc_rows,
-                    &en->outbuf, &en->qmap, &en->slot_src, &en->prof_out};
+                    &en->outbuf, &en->qmap, &en->slot_src};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
   if (en->d_ctl) (void)hipFree(en->d_ctl);

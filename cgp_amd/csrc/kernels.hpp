@@ -17,9 +17,10 @@
 //                                      degree-binned frontier work lists,
 //                                      double-buffered by iteration parity
 //
-// One iteration = push kernels (lane-per-edge over the three degree bins for
-// BFS bits and sparse SSSP words; lane-per-query waves for dense SSSP words),
-// an optional BFS pull kernel, then the filter kernel that claims next words,
+// One iteration = one advance launch (push lane-per-edge over the three
+// degree bins for BFS bits and sparse SSSP words, lane-per-query waves for
+// dense SSSP words, and on pull iterations a bottom-up BFS pass over all
+// vertices), then the filter launch that claims next words,
 // settles BFS depths, and emits the next binned frontier with wave-level
 // prefix sums.  Every kernel reads its work counts from the device control
 // block, so a sequence of iterations can be captured in a hipGraph and
@@ -54,6 +55,7 @@ struct Cfg {
   u64 pull_pairs_thr;   // hybrid: pull when BFS frontier (vertex,query) pairs >= this
   u64 pull_edges_thr;   // ... or when their edge-weighted count >= this
   u64 push_pairs_thr;   // back to push when pairs fall below this
+  u64 dense_fedges_thr; // dense frontier mode when the next frontier's edges reach this
 };
 
 // Device control block: counters and the iteration state machine.
@@ -67,13 +69,22 @@ struct Ctl {
   u32 max_iter;     // 0 = unlimited
   u32 push_iters;
   u32 pull_iters;
-  u32 pad[3];
+  u32 dense_mode;   // 1: advance uses fire-and-forget atomics, filter scans all vertices
+  u32 dense_iters;
+  u32 pad;
   u32 bcount[2][4];           // [parity][small, medium, large items, dense items]
+  u32 fetch[2][4];            // [parity] dynamic work cursors of the advance (L, D, M, S)
   u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
   u64 bfs_edges[2];           // ... weighted by degree
+  u64 fedges[2];              // summed degree of the next frontier's vertices
   u64 union_[MAXW];           // OR of next words in the running filter
   u64 last_union[MAXW];       // OR of the frontier words the last filter produced
   u64 prof[NPROF];            // work counters (profile mode)
+  // Device-clock launch timing (wall_clock64 ticks): first workgroup start
+  // and last workgroup end of the running advance / filter, and the sums
+  // over the launches of the run that did work.
+  u64 adv_t0, adv_t1, flt_t0;
+  u64 adv_ticks, flt_ticks, adv_launches;
 };
 
 // Static per-engine kernel parameters (buffer addresses and tuning).

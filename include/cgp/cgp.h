@@ -80,7 +80,8 @@ typedef struct {
   int32_t bin_large;           /* degree > bin_large: one workgroup per vertex; 0 = tuned */
   int32_t use_hip_graph;       /* capture iterations into a hipGraph (default 1) */
   int32_t graph_iterations;    /* iterations per captured graph launch (default 8) */
-  int32_t profile;             /* 1 = per-launch HIP events and device work counters */
+  int32_t profile;             /* bit 0: per-launch HIP events (also inside graph replays);
+                                  bit 1: device work counters -> advance_bytes */
   uint64_t hbm_budget_bytes;   /* serving: cap on per-query value memory; 0 = 64 GiB */
 } cgp_engine_opts;
 
@@ -92,15 +93,15 @@ typedef struct {
   int32_t pull_iterations;
   int32_t passes;              /* word-group passes (batch split by max_words) */
   double wall_ms;              /* host wall, first launch -> results resident on the device */
-  double init_ms;              /* profile only: init kernels */
-  double advance_ms;           /* profile only: summed push+pull advance kernel time */
-  double filter_ms;            /* profile only: summed compaction kernel time */
-  double wcc_ms;               /* profile only: hooking + pointer jumping */
-  double output_ms;            /* profile only: result transpose */
-  int64_t advance_launches;    /* profile only: advance launches that did work */
-  double advance_bytes;        /* profile only: algorithmic bytes of all advance launches */
-  int64_t frontier_vertices;   /* profile only: summed frontier sizes */
-  int64_t edges_scanned;       /* profile only: edges touched by push and pull */
+  double init_ms;              /* profile bit 0: init kernels */
+  double advance_ms;           /* profile bit 0: summed advance kernel time */
+  double filter_ms;            /* profile bit 0: summed compaction kernel time */
+  double wcc_ms;               /* profile bit 0: hooking + pointer jumping */
+  double output_ms;            /* profile bit 0: result transpose */
+  int64_t advance_launches;    /* profile bit 0: advance launches that did work */
+  double advance_bytes;        /* profile bit 1: algorithmic bytes of all advance launches */
+  int64_t frontier_vertices;   /* profile bit 1: frontier items expanded */
+  int64_t edges_scanned;       /* profile bit 1: edges touched by push and pull */
 } cgp_run_stats;
 
 /* ---------------------------------------------------------------- errors */
@@ -122,7 +123,7 @@ cgp_status cgp_copy_to_device(void* dst_device, const void* src_host, uint64_t b
 /* RMAT (a,b,c, d = 1-a-b-c) with 2^scale vertices and edge_factor * 2^scale edge draws,
  * self-loops dropped, symmetrised, deduplicated, sorted; weights uniform in
  * [wmin, wmax] from a hash of (min(u,v), max(u,v), seed) so both directions agree.
- * wmax == 0 produces an unweighted graph. Has no reference counterpart. */
+ * wmax == 0 produces an unweighted graph. The reference path has no generator. */
 cgp_status cgp_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
                         uint64_t seed, uint32_t wmin, uint32_t wmax, cgp_host_csr** out);
 /* rows x cols grid, 4-neighbour undirected edges, plus per vertex with probability

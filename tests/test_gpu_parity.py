@@ -103,7 +103,7 @@ def test_rmat_wcc(gpu):
 
 
 @pytest.mark.parametrize("hip_graph", [0, 1])
-@pytest.mark.parametrize("profile", [0, 1])
+@pytest.mark.parametrize("profile", [0, 1, 3])
 def test_mixed_uneven_batch(gpu, hip_graph, profile):
     # 160 BFS + 80 SSSP + 16 WCC in one batch (4 mask words, a mixed word),
     # shuffled order; results must equal the oracle row by row.
@@ -120,8 +120,11 @@ def test_mixed_uneven_batch(gpu, hip_graph, profile):
     exp = oracle_rows(og, kinds, sources)
     assert np.array_equal(out, exp)
     assert st["passes"] == 1
-    if profile:
-        assert st["advance_ms"] > 0 and st["advance_bytes"] > 0 and st["edges_scanned"] > 0
+    if profile & 1:
+        assert st["advance_ms"] > 0 and st["advance_launches"] == st["iterations"]
+        assert st["filter_ms"] > 0
+    if profile & 2:
+        assert st["advance_bytes"] > 0 and st["edges_scanned"] > 0
 
 
 def test_alone_equals_in_the_mix(gpu):
