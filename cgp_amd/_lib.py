@@ -43,6 +43,8 @@ class cgp_engine_opts(C.Structure):
         ("graph_iterations", C.c_int32),
         ("profile", C.c_int32),
         ("hbm_budget_bytes", C.c_uint64),
+        ("dense_threshold", C.c_int32),
+        ("frontier_dense_div", C.c_int32),
     ]
 
 

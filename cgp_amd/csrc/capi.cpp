@@ -15,21 +15,6 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
 }  // namespace cgpi
 
-#define CGP_TRY(expr)                                  \
-  try {                                                \
-    expr;                                              \
-    return CGP_OK;                                     \
-  } catch (const cgpi::Error& e_) {                    \
-    cgpi::set_last_error(e_.what());                   \
-    return e_.status;                                  \
-  } catch (const std::bad_alloc&) {                    \
-    cgpi::set_last_error("host allocation failed");    \
-    return CGP_ENOMEM;                                 \
-  } catch (const std::exception& e_) {                 \
-    cgpi::set_last_error(e_.what());                   \
-    return CGP_EINVAL;                                 \
-  }
-
 namespace {
 cgp_status wrap_host(cgpi::HostCSR&& g, cgp_host_csr** out) {
   std::unique_ptr<cgp_host_csr> h(new cgp_host_csr());

@@ -77,6 +77,23 @@ bool is_symmetric(const HostCSR& g);
 
 }  // namespace cgpi
 
+// Body wrapper of every C-ABI entry point: exceptions become a status plus
+// the thread-local last-error message.
+#define CGP_TRY(expr)                                  \
+  try {                                                \
+    expr;                                              \
+    return CGP_OK;                                     \
+  } catch (const cgpi::Error& e_) {                    \
+    cgpi::set_last_error(e_.what());                   \
+    return e_.status;                                  \
+  } catch (const std::bad_alloc&) {                    \
+    cgpi::set_last_error("host allocation failed");    \
+    return CGP_ENOMEM;                                 \
+  } catch (const std::exception& e_) {                 \
+    cgpi::set_last_error(e_.what());                   \
+    return CGP_EINVAL;                                 \
+  }
+
 struct cgp_host_csr {
   cgpi::HostCSR g;
 };

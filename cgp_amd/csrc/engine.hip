@@ -23,18 +23,10 @@
 #include <string>
 #include <vector>
 
-#include "cgp_internal.hpp"
-#include "kernels.hpp"
+#include "engine_impl.hpp"
 
 using namespace cgpk;
-
-#define HIP_CHECK(x)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (x);                                                              \
-    if (e_ != hipSuccess)                                                             \
-      cgpi::fail(e_ == hipErrorOutOfMemory ? CGP_ENOMEM : CGP_EDEVICE,                \
-                 std::string(#x) + ": " + hipGetErrorString(e_));                     \
-  } while (0)
+using namespace cgpi;
 
 // ============================================================== device helpers
 namespace {
@@ -225,43 +217,145 @@ __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0,
   }
 }
 
-// Dense SSSP word w of vertex u over edges [e0, e1): one wave, lane q is
-// query q of the word, edges walked in order with 8 value rows in flight per
-// step; each value access is one coalesced row.  Every lane of the wave calls it.
+// Wave-level inclusive prefix sum (every lane of the wave calls it).
+__device__ __forceinline__ u32 wave_incl_scan(u32 x) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 y = (u32)__shfl_up((int)x, d, 64);
+    if (lane_id() >= d) x += y;
+  }
+  return x;
+}
+
+// Index of the first lane whose inclusive prefix `incl` exceeds f (lanes
+// hold a non-decreasing sequence; every lane of the wave calls it).
+__device__ __forceinline__ int wave_upper_bound(u32 incl, u32 f) {
+  int lo = 0, hi = 63;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int mid = (lo + hi) >> 1;
+    const u32 x = (u32)__shfl((int)incl, mid, 64);
+    if (x > f) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Phase S: a wave takes up to 64 low-degree frontier vertices at once; the
+// lanes load their metadata in parallel, a wave prefix sum over the degrees
+// flattens their edge lists, and each step relaxes 64 consecutive flattened
+// edges, one per lane (load-balanced lane-per-edge).
 template <int W, bool SPARSE>
-__device__ __forceinline__ void expand_dense(const KP& P, int32_t u, int w, int32_t e0, int32_t e1,
-                                             int rshift, u32 stamp, Tally& t) {
+__device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 base, u32 n,
+                                                     bool bfs_on, int dense_t, int rshift, u32 stamp,
+                                                     Tally& t) {
+  const int lane = lane_id();
+  const bool have = base + (u32)lane < n;
+  const int32_t u = have ? list[base + lane] : 0;
+  u64 bm[W], sm[W];
+  bool any_s = false, any_b = false;
+  if (have) load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (!have) bm[w] = sm[w] = 0;
+    any_b |= bm[w] != 0;
+  }
+  const bool work = have && (any_b || any_s);
+  const int32_t e0 = work ? P.off[u] : 0;
+  const u32 d = work ? (u32)(P.off[u + 1] - e0) : 0u;
+  if (work) ++t.items;
+  const u32 incl = wave_incl_scan(d);
+  const u32 total = (u32)__shfl((int)incl, 63, 64);
+  for (u32 win = 0; win < total; win += 64) {
+    const u32 f = win + (u32)lane;
+    const int j = wave_upper_bound(incl, f);
+    const int32_t uj = __shfl(u, j, 64);
+    const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
+    const int32_t ej = __shfl(e0, j, 64) + (int32_t)(f - exj);
+    u64 bj[W], sj[W];
+    bool sj_any = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      bj[w] = shfl64(bm[w], j);
+      sj[w] = shfl64(sm[w], j);
+      sj_any |= sj[w] != 0;
+    }
+    bool claimed = false;
+    int32_t v = 0;
+    if (f < total) {
+      v = P.idx[ej];
+      u32 wv = 1;
+      if (sj_any && P.wt) {
+        wv = P.wt[ej];
+        ++t.wedges;
+      }
+      ++t.edges;
+      claimed = relax_edge<W, SPARSE>(P, uj, v, wv, bj, sj, stamp, rshift, t.bfs, t.sssp, t.nnew, t.imp);
+    }
+    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
+  }
+}
+
+// Phase D: a wave takes up to 64 dense (vertex, word, chunk) items at once,
+// flattens their edges as above, and walks them 8 per step with lane q as
+// query q: each step keeps 16 value rows in flight (8 source rows, mostly
+// cache hits, and 8 target rows), so no per-item latency chain remains.
+template <int W, bool SPARSE>
+__device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
+                                                  Tally& t) {
   constexpr int U = 8;
   const int lane = lane_id();
   const int R = 1 << rshift;
-  const u64 sm = P.front[(size_t)u * W + w] & P.cfg->val_bits[w];
-  const bool act = lane < R && ((sm >> lane) & 1ull);
-  const u32 du = act ? P.vals[vrow(P.V, w, u, rshift) + lane] : INF;
-  if (lane == 0) {
+  const u32 p = P.ctl->iter & 1u;
+  const bool have = base + (u32)lane < n;
+  const u64 item = have ? P.dense[p][base + lane] : 0ull;
+  const int32_t u = (int32_t)(item >> 32);
+  const int w = (int)((item >> 24) & 0xFF);
+  const int32_t e0 = have ? P.off[u] + (int32_t)((u32)(item & 0xFFFFFF) * DCHUNK) : 0;
+  const int32_t e1 = have ? min(P.off[u + 1], e0 + DCHUNK) : 0;
+  const u64 sm = have ? (P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) : 0ull;
+  const u32 d = (u32)(e1 - e0);
+  if (have) {
     ++t.ditems;
-    t.dedges += (u64)(e1 - e0);
-    t.sssp += (u64)(e1 - e0) * (u64)__popcll(sm);
+    t.dedges += d;
+    t.sssp += (u64)d * (u64)__popcll(sm);
   }
-  for (int32_t eb = e0; eb < e1; eb += 64) {
-    const int cnt = min(64, e1 - eb);
-    const int32_t myv = lane < cnt ? P.idx[eb + lane] : 0;
-    const u32 myw = lane < cnt ? (P.wt ? P.wt[eb + lane] : 1u) : 0u;
-    for (int j = 0; j < cnt; j += U) {
-      int32_t vv[U];
+  const u32 incl = wave_incl_scan(d);
+  const u32 total = (u32)__shfl((int)incl, 63, 64);
+  const u32 smlo = (u32)sm, smhi = (u32)(sm >> 32);
+  for (u32 win = 0; win < total; win += 64) {
+    const u32 f = win + (u32)lane;
+    const int j = wave_upper_bound(incl, f);
+    const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
+    const int32_t ej = __shfl(e0, j, 64) + (int32_t)(f - exj);
+    const bool ok = f < total;
+    const int32_t myv = ok ? P.idx[ej] : 0;
+    const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : 0u;
+    const int cnt = (int)min(64u, total - win);
+    for (int k0 = 0; k0 < cnt; k0 += U) {
+      int32_t vv[U], ww[U];
       u32 cand[U], cur[U];
+      bool act[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        const int jj = min(j + k, cnt - 1);
+        const int jj = min(k0 + k, cnt - 1);
         vv[k] = __builtin_amdgcn_readlane(myv, jj);
         const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, jj);
+        const int it = __builtin_amdgcn_readlane(j, jj);
+        const int32_t uk = __builtin_amdgcn_readlane(u, it);
+        ww[k] = __builtin_amdgcn_readlane(w, it);
+        const u64 smk = ((u64)(u32)__builtin_amdgcn_readlane((int)smhi, it) << 32) |
+                        (u32)__builtin_amdgcn_readlane((int)smlo, it);
+        act[k] = (k0 + k < cnt) && lane < R && ((smk >> lane) & 1ull);
+        const u32 du = act[k] ? P.vals[vrow(P.V, ww[k], uk, rshift) + lane] : INF;
         cand[k] = du + wk;
-        if (cand[k] < du || j + k >= cnt) cand[k] = INF;
-        cur[k] = (act && j + k < cnt) ? P.vals[vrow(P.V, w, vv[k], rshift) + lane] : 0u;
+        if (cand[k] < du) cand[k] = INF;
+        cur[k] = act[k] ? P.vals[vrow(P.V, ww[k], vv[k], rshift) + lane] : 0u;
       }
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        const bool imp = act && cand[k] < cur[k];
-        if (imp) atomicMin(&P.vals[vrow(P.V, w, vv[k], rshift) + lane], cand[k]);
+        const bool imp = act[k] && cand[k] < cur[k];
+        if (imp) atomicMin(&P.vals[vrow(P.V, ww[k], vv[k], rshift) + lane], cand[k]);
         const u64 m = __ballot(imp);
         if (m) {
           t.imp += imp ? 1 : 0;
@@ -269,10 +363,10 @@ __device__ __forceinline__ void expand_dense(const KP& P, int32_t u, int w, int3
           if (lane == 0) {
             ++t.nnew;
             if (SPARSE) {
-              if (atomicOr(&P.next[(size_t)vv[k] * W + w], m) == 0)
+              if (atomicOr(&P.next[(size_t)vv[k] * W + ww[k]], m) == 0)
                 claimed = claim_vertex(P.mark, vv[k], stamp);
             } else {
-              atomicOr(&P.next[(size_t)vv[k] * W + w], m);
+              atomicOr(&P.next[(size_t)vv[k] * W + ww[k]], m);
             }
           }
           if (SPARSE) wave_append<int32_t>(claimed, vv[k], &P.ctl->tcount, P.touched);
@@ -340,11 +434,12 @@ __device__ __forceinline__ u32 wave_fetch(u32* ctr, u32 n) {
 // Body of the advance for one frontier mode; workgroups pull work
 // dynamically, heaviest first:
 //   L  one workgroup per LCHUNK-edge chunk of a high-degree vertex
-//   D  one wave per (vertex, dense SSSP word, DCHUNK-edge chunk), lane = query
+//   D  64 (vertex, dense SSSP word, DCHUNK-edge chunk) items per wave,
+//      edges flattened, lane = query
 //   M  one wave per medium-degree vertex, lane = edge
-//   S  8 lanes per low-degree vertex, lane = edge
+//   S  64 low-degree vertices per wave, edges flattened, lane = edge
 //   P  (pull iterations) one lane per vertex of the whole graph
-template <int W, bool SPARSE>
+template <int W, bool SPARSE, int BS>
 __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t) {
   Ctl* C = P.ctl;
   const u32 iter = C->iter, p = iter & 1u;
@@ -364,20 +459,15 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
     const int32_t u = (int32_t)(item >> 32);
     const int32_t e0 = P.off[u] + (int32_t)((u32)item * LCHUNK);
     const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
-    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BLOCK, bfs_on, dense_t, rshift,
+    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, dense_t, rshift,
                             stamp, t);
   }
   // Phase D.
   const u32 nD = C->bcount[p][3];
   while (nD) {
-    const u32 i = wave_fetch(&C->fetch[p][1], 1u);
+    const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
-    const u64 item = P.dense[p][i];
-    const int32_t u = (int32_t)(item >> 32);
-    const int w = (int)((item >> 24) & 0xFF);
-    const int32_t e0 = P.off[u] + (int32_t)((u32)(item & 0xFFFFFF) * DCHUNK);
-    const int32_t e1 = min(P.off[u + 1], e0 + DCHUNK);
-    expand_dense<W, SPARSE>(P, u, w, e0, e1, rshift, stamp, t);
+    expand_flat_dense<W, SPARSE>(P, i, nD, rshift, stamp, t);
   }
   // Phase M.
   const u32 nM = C->bcount[p][1];
@@ -391,21 +481,16 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
   // Phase S.
   const u32 nS = C->bcount[p][0];
   while (nS) {
-    const u32 base = wave_fetch(&C->fetch[p][3], 8u);
+    const u32 base = wave_fetch(&C->fetch[p][3], 64u);
     if (base >= nS) break;
-    const u32 i = base + (u32)(lane >> 3);
-    if (i < nS) {
-      const int32_t u = P.small[p][i];
-      const int32_t e0 = P.off[u], e1 = P.off[u + 1];
-      expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (lane & 7), 8, bfs_on, dense_t, rshift, stamp, t);
-    }
+    expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, t);
   }
   // Phase P.
   if (!bfs_on) {
     u64 bb[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
-    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < P.V; base += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t base = (int64_t)blockIdx.x * BS; base < P.V; base += (int64_t)gridDim.x * BS) {
       const int32_t v = (int32_t)(base + threadIdx.x);
       const bool claimed = v < P.V && pull_vertex<W, SPARSE>(P, v, bb, stamp, t);
       if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
@@ -413,16 +498,17 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
   }
 }
 
-// The advance of one iteration, as one persistent launch.
-template <int W>
-__global__ __launch_bounds__(BLOCK) void k_advance(KP P) {
+// The advance of one iteration, as one persistent launch of BS-thread
+// workgroups (BS is a tuning parameter: 128, 256 or 512).
+template <int W, int BS>
+__global__ __launch_bounds__(BS) void k_advance(KP P) {
   Ctl* C = P.ctl;
   if (C->finished) return;
   __shared__ u32 s_item;
   if (threadIdx.x == 0) atomicMin(&C->adv_t0, (u64)wall_clock64());
   Tally t;
-  if (C->dense_mode) advance_body<W, false>(P, &s_item, t);
-  else advance_body<W, true>(P, &s_item, t);
+  if (C->dense_mode) advance_body<W, false, BS>(P, &s_item, t);
+  else advance_body<W, true, BS>(P, &s_item, t);
   if (P.profile) {
     prof_add(P, PF_PUSH_ITEMS, t.items);
     prof_add(P, PF_PUSH_EDGES, t.edges);
@@ -449,7 +535,9 @@ template <int W>
 __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, int nslots) {
   Ctl* C = P.ctl;
   const int s = (int)(blockIdx.x * BLOCK + threadIdx.x);
-  const u32 stamp = C->stamp_base;  // stamp of iteration 0xFFFFFFFF
+  // Stamp of the iteration whose filter settles the seeds (stamp_base itself
+  // for the batch seeding pass, which runs at iteration 0xFFFFFFFF).
+  const u32 stamp = C->stamp_base + C->iter + 1u;
   bool claimed = false;
   int32_t v = 0;
   if (s < nslots && slot_src[s] >= 0) {
@@ -758,202 +846,17 @@ __global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, int32_t V
 }  // namespace
 
 // ============================================================== host side
-struct cgp_graph {
-  int32_t V = 0;
-  int64_t E = 0;
-  bool symmetric = false;
-  bool weighted = false;
-  int32_t* off = nullptr;
-  int32_t* idx = nullptr;
-  int32_t* rows = nullptr;
-  u32* wt = nullptr;
-};
-
 namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  // Grows to at least `need` bytes; returns true when the address changed.
-  bool ensure(size_t need) {
-    if (need <= bytes) return false;
-    release();
-    if (need == 0) return false;
-    HIP_CHECK(hipMalloc(&p, need));
-    bytes = need;
-    return true;
-  }
-};
-
-int device_cus() {
-  int dev = 0;
-  HIP_CHECK(hipGetDevice(&dev));
-  int cus = 0;
-  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  return cus > 0 ? cus : 1;
-}
-
-void require_device() {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
-    cgpi::fail(CGP_EDEVICE, "no usable HIP device (the engine needs a gfx950 GPU)");
-}
-
-double now_ms() {
-  return std::chrono::duration<double, std::milli>(
-             std::chrono::steady_clock::now().time_since_epoch())
-      .count();
-}
-
-int round_words(int w) { return w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : 8; }
-
-}  // namespace
-
-struct cgp_engine {
-  const cgp_graph* g = nullptr;
-  cgp_engine_opts opts{};
-  hipStream_t s = nullptr;
-  int cus = 1;
-  int wcap = 0;            // words allocated in visited/next/front/dense
-  DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
-  DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
-  Ctl* d_ctl = nullptr;
-  Cfg* d_cfg = nullptr;
-  Ctl* h_ctl = nullptr;     // pinned
-  Cfg* h_cfg = nullptr;     // pinned
-  int32_t* h_stage = nullptr;  // pinned staging (slot sources, qmap, wcc rows)
-  size_t h_stage_cap = 0;
-  u32 stamp_epoch = 1;
-  bool next_dirty = false;
-  bool busy = false;
-  std::mutex mu;
-  // Captured iteration graphs, one per mask-word width.
-  struct GExec {
-    hipGraphExec_t exec = nullptr;
-    int K = 0;
-    int prof = -1;
-  };
-  GExec gx[4];
-  double clock_khz = 100000.0;  // device wall-clock rate (wall_clock64 ticks)
-  // Per-phase timing of init / WCC / output (launch timing on, outside graphs).
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  bool timed() const { return (opts.profile & 1) != 0; }
-  bool counted() const { return (opts.profile & 2) != 0; }
-  // Serving session state lives in serve.hip (attached here).
-  void* serve = nullptr;
-
-  KP kp() const {
-    KP P{};
-    P.V = g->V;
-    P.E = g->E;
-    P.profile = (opts.profile & 2) ? 1 : 0;
-    P.off = g->off;
-    P.idx = g->idx;
-    P.wt = g->weighted ? g->wt : nullptr;
-    P.rows = g->rows;
-    P.visited = (u64*)visited.p;
-    P.next = (u64*)next.p;
-    P.front = (u64*)front.p;
-    P.mark = (u32*)mark.p;
-    P.vals = (u32*)vals.p;
-    P.touched = (int32_t*)touched.p;
-    for (int k = 0; k < 2; ++k) {
-      P.small[k] = (int32_t*)small[k].p;
-      P.medium[k] = (int32_t*)medium[k].p;
-      P.large[k] = (u64*)large[k].p;
-      P.dense[k] = (u64*)dense[k].p;
-    }
-    P.ctl = d_ctl;
-    P.cfg = d_cfg;
-    P.bin_small = opts.bin_small;
-    P.bin_large = opts.bin_large;
-    return P;
-  }
-
-  void drop_graphs() {
-    for (int k = 0; k < 4; ++k) {
-      if (gx[k].exec) (void)hipGraphExecDestroy(gx[k].exec);
-      gx[k].exec = nullptr;
-      gx[k].K = 0;
-      gx[k].prof = -1;
-    }
-  }
-
-  // Grid sizes: persistent-style, sized to the device, independent of the
-  // frontier so the launches can be captured once and replayed.
-  int grid_advance() const { return cus * 8; }
-  int grid_filter() const {
-    int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
-  }
-
-  void ensure_words(int W) {
-    const size_t V = (size_t)g->V, E = (size_t)g->E;
-    bool moved = false;
-    if (W > wcap) {
-      moved |= visited.ensure(V * W * 8);
-      moved |= next.ensure(V * W * 8);
-      moved |= front.ensure(V * W * 8);
-      HIP_CHECK(hipMemsetAsync(next.p, 0, V * W * 8, s));
-      for (int k = 0; k < 2; ++k) moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
-      wcap = W;
-    }
-    if (!mark.p) {
-      moved |= mark.ensure(V * 4 + 4);
-      HIP_CHECK(hipMemsetAsync(mark.p, 0, V * 4 + 4, s));
-      moved |= touched.ensure(V * 4 + 4);
-      for (int k = 0; k < 2; ++k) {
-        moved |= small[k].ensure(V * 4 + 4);
-        moved |= medium[k].ensure(V * 4 + 4);
-        moved |= large[k].ensure((V + E / LCHUNK + 1) * 8);
-      }
-    }
-    if (moved) drop_graphs();
-  }
-
-  void ensure_vals(size_t elems) {
-    if (vals.ensure(elems * 4)) drop_graphs();
-  }
-
-  void stage(size_t ints) {
-    if (ints <= h_stage_cap) return;
-    if (h_stage) (void)hipHostFree(h_stage);
-    h_stage = nullptr;
-    HIP_CHECK(hipHostMalloc((void**)&h_stage, ints * 4, hipHostMallocDefault));
-    h_stage_cap = ints;
-  }
-
-  // ---- timing helpers (launch timing on)
-  void tick(int cat, bool begin) {
-    if (!timed()) return;
-    if (ev_used >= ev_pool.size()) {
-      hipEvent_t e;
-      HIP_CHECK(hipEventCreate(&e));
-      ev_pool.push_back(e);
-    }
-    hipEvent_t e = ev_pool[ev_used++];
-    HIP_CHECK(hipEventRecord(e, s));
-    if (begin) ev.push_back({cat, {e, nullptr}});
-    else ev.back().second.second = e;
-  }
-};
-
-namespace {
-
-enum { CAT_INIT = 0, CAT_ADV = 1, CAT_FILTER = 2, CAT_WCC = 3, CAT_OUT = 4 };
 
 // One iteration: advance then filter (both time themselves on the device clock).
 template <int W>
 void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
-  hipLaunchKernelGGL((k_advance<W>), dim3(en->grid_advance()), dim3(BLOCK), 0, s, P);
+  const int bs = en->opts.block_size;
+  const dim3 grid(en->grid_advance());
+  if (bs == 128) hipLaunchKernelGGL((k_advance<W, 128>), grid, dim3(128), 0, s, P);
+  else if (bs == 512) hipLaunchKernelGGL((k_advance<W, 512>), grid, dim3(512), 0, s, P);
+  else hipLaunchKernelGGL((k_advance<W, 256>), grid, dim3(256), 0, s, P);
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
   HIP_CHECK(hipGetLastError());
 }
@@ -966,12 +869,32 @@ void launch_seed(cgp_engine* en, const KP& P, const int32_t* d_slot_src, int nsl
   HIP_CHECK(hipGetLastError());
 }
 
+template <int W>
+void launch_seed_only(cgp_engine* en, const KP& P, const int32_t* d_slot_src, int nslots) {
+  hipLaunchKernelGGL((k_seed<W>), dim3((nslots + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, en->s, P,
+                     d_slot_src, nslots);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+namespace cgpi {
+
 void dispatch_iteration(cgp_engine* en, int W, const KP& P) {
   switch (W) {
     case 1: launch_iteration<1>(en, P); break;
     case 2: launch_iteration<2>(en, P); break;
     case 4: launch_iteration<4>(en, P); break;
     default: launch_iteration<8>(en, P); break;
+  }
+}
+
+void dispatch_seed_only(cgp_engine* en, int W, const KP& P, const int32_t* src, int nslots) {
+  switch (W) {
+    case 1: launch_seed_only<1>(en, P, src, nslots); break;
+    case 2: launch_seed_only<2>(en, P, src, nslots); break;
+    case 4: launch_seed_only<4>(en, P, src, nslots); break;
+    default: launch_seed_only<8>(en, P, src, nslots); break;
   }
 }
 
@@ -991,12 +914,29 @@ void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
   HIP_CHECK(hipGetLastError());
 }
 
-int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
+static int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
+
+void compute_wcc(cgp_engine* en) {
+  const cgp_graph* g = en->g;
+  en->parent.ensure((size_t)g->V * 4 + 4);
+  const int gb = en->cus * 16;
+  hipLaunchKernelGGL(k_wcc_init, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, g->V);
+  hipLaunchKernelGGL(k_wcc_hook, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->rows,
+                     (const int32_t*)g->idx, g->E, (int32_t*)en->parent.p, g->symmetric ? 1 : 0);
+  hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, g->V);
+  HIP_CHECK(hipGetLastError());
+}
+
+void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) {
+  hipLaunchKernelGGL(k_wcc_rows, dim3(en->cus * 16), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p,
+                     en->g->V, d_rows, nrows, out);
+  HIP_CHECK(hipGetLastError());
+}
 
 // Runs the frontier iterations of one pass to convergence: replays of a
 // captured K-iteration graph (or direct launches), one host readback of the
 // control block per replay.
-void run_iterations(cgp_engine* en, int W) {
+static void run_iterations(cgp_engine* en, int W) {
   const KP P = en->kp();
   const bool use_graph = en->opts.use_hip_graph != 0;
   const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
@@ -1031,10 +971,7 @@ void run_iterations(cgp_engine* en, int W) {
   }
 }
 
-}  // namespace
 
-// ------------------------------------------------------------------ batch run
-namespace cgpi {
 
 // Algorithmic bytes of the advance launches of one pass (each byte at its
 // natural width, counted once, no cache-line waste):
@@ -1110,7 +1047,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     std::memset(&cfg, 0, sizeof(Cfg));
     cfg.rshift = rshift;
     cfg.W = W;
-    cfg.dense_t = 16;
+    cfg.dense_t = en->opts.dense_threshold;
     en->stage((size_t)W * 64 * 2);
     int32_t* h_src = en->h_stage;
     int32_t* h_qmap = en->h_stage + W * 64;
@@ -1139,7 +1076,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH &&
                           (en->opts.traversal_mode == CGP_PULL || nb <= kPullMaxQueries);
     cfg.pull_policy = !can_pull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
-    cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / 8);
+    cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
     cfg.pull_pairs_thr = (u64)(en->opts.pull_frontier_ratio * (double)nb * (double)V);
     cfg.pull_edges_thr = (u64)(en->opts.pull_edge_ratio * (double)nb * (double)g->E);
     cfg.push_pairs_thr = cfg.pull_pairs_thr / 4 + 1;
@@ -1193,23 +1130,16 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->tick(CAT_OUT, false);
   }
   if (!wcc.empty()) {
-    en->parent.ensure((size_t)V * 4 + 4);
     en->wcc_rows.ensure(wcc.size() * 4);
     en->stage(wcc.size());
     std::memcpy(en->h_stage, wcc.data(), wcc.size() * 4);
     HIP_CHECK(hipMemcpyAsync(en->wcc_rows.p, en->h_stage, wcc.size() * 4, hipMemcpyHostToDevice, en->s));
-    const int gb = en->cus * 16;
     en->tick(CAT_WCC, true);
-    hipLaunchKernelGGL(k_wcc_init, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, V);
-    hipLaunchKernelGGL(k_wcc_hook, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->rows,
-                       (const int32_t*)g->idx, g->E, (int32_t*)en->parent.p, g->symmetric ? 1 : 0);
-    hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, V);
+    compute_wcc(en);
     en->tick(CAT_WCC, false);
     en->tick(CAT_OUT, true);
-    hipLaunchKernelGGL(k_wcc_rows, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p, V,
-                       (const int32_t*)en->wcc_rows.p, (int)wcc.size(), dout);
+    write_wcc_rows(en, (const int32_t*)en->wcc_rows.p, (int)wcc.size(), dout);
     en->tick(CAT_OUT, false);
-    HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipStreamSynchronize(en->s));
   st.wall_ms = now_ms() - t0;
@@ -1233,21 +1163,6 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
 }  // namespace cgpi
 
 // ================================================================= C-ABI (device side)
-#define CGP_TRY(expr)                                  \
-  try {                                                \
-    expr;                                              \
-    return CGP_OK;                                     \
-  } catch (const cgpi::Error& e_) {                    \
-    cgpi::set_last_error(e_.what());                   \
-    return e_.status;                                  \
-  } catch (const std::bad_alloc&) {                    \
-    cgpi::set_last_error("host allocation failed");    \
-    return CGP_ENOMEM;                                 \
-  } catch (const std::exception& e_) {                 \
-    cgpi::set_last_error(e_.what());                   \
-    return CGP_EINVAL;                                 \
-  }
-
 extern "C" {
 
 cgp_status cgp_device_count(int32_t* count) {
@@ -1366,10 +1281,12 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->pull_edge_ratio = 0.20;
   o->max_iterations = 0;
   o->max_words = MAXW;
-  o->block_size = BLOCK;
-  o->bin_small = 16;
+  o->block_size = 256;
+  o->bin_small = 64;
   o->bin_large = 2048;
   o->use_hip_graph = 1;
+  o->dense_threshold = 16;
+  o->frontier_dense_div = 8;
   o->graph_iterations = 8;
   o->profile = 0;
   o->hbm_budget_bytes = 64ull << 30;
@@ -1400,7 +1317,10 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
       en->opts = d;
     }
     if (en->opts.bin_large < en->opts.bin_small) en->opts.bin_large = en->opts.bin_small;
-    en->opts.block_size = BLOCK;
+    if (en->opts.block_size != 128 && en->opts.block_size != 256 && en->opts.block_size != 512)
+      cgpi::fail(CGP_EINVAL, "cgp_engine_create: block_size must be 128, 256 or 512");
+    if (en->opts.dense_threshold <= 0) en->opts.dense_threshold = d.dense_threshold;
+    if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
     en->cus = device_cus();
     {
       int dev = 0;

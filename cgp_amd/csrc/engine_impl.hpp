@@ -1,0 +1,243 @@
+// engine_impl.hpp — host-side engine state shared by engine.hip (batch
+// runs) and serve.hip (continuous batching).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cgp_internal.hpp"
+#include "kernels.hpp"
+
+using cgpk::Cfg;
+using cgpk::Ctl;
+using cgpk::KP;
+using cgpk::u32;
+using cgpk::u64;
+using cgpk::BLOCK;
+using cgpk::DCHUNK;
+using cgpk::LCHUNK;
+using cgpk::MAXW;
+using cgpk::INF;
+
+#define HIP_CHECK(x)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      cgpi::fail(e_ == hipErrorOutOfMemory ? CGP_ENOMEM : CGP_EDEVICE,                \
+                 std::string(#x) + ": " + hipGetErrorString(e_));                     \
+  } while (0)
+
+struct cgp_graph {
+  int32_t V = 0;
+  int64_t E = 0;
+  bool symmetric = false;
+  bool weighted = false;
+  int32_t* off = nullptr;
+  int32_t* idx = nullptr;
+  int32_t* rows = nullptr;
+  u32* wt = nullptr;
+};
+
+namespace cgpi {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grows to at least `need` bytes; returns true when the address changed.
+  bool ensure(size_t need) {
+    if (need <= bytes) return false;
+    release();
+    if (need == 0) return false;
+    HIP_CHECK(hipMalloc(&p, need));
+    bytes = need;
+    return true;
+  }
+};
+
+inline int device_cus() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  int cus = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus > 0 ? cus : 1;
+}
+
+inline void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    cgpi::fail(CGP_EDEVICE, "no usable HIP device (the engine needs a gfx950 GPU)");
+}
+
+inline double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+inline int round_words(int w) { return w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : 8; }
+
+}  // namespace cgpi
+
+struct ServeState;
+
+struct cgp_engine {
+  const cgp_graph* g = nullptr;
+  cgp_engine_opts opts{};
+  hipStream_t s = nullptr;
+  int cus = 1;
+  int wcap = 0;            // words allocated in visited/next/front/dense
+  cgpi::DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
+  cgpi::DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
+  Ctl* d_ctl = nullptr;
+  Cfg* d_cfg = nullptr;
+  Ctl* h_ctl = nullptr;     // pinned
+  Cfg* h_cfg = nullptr;     // pinned
+  int32_t* h_stage = nullptr;  // pinned staging (slot sources, qmap, wcc rows)
+  size_t h_stage_cap = 0;
+  u32 stamp_epoch = 1;
+  bool next_dirty = false;
+  bool busy = false;
+  std::mutex mu;
+  // Captured iteration graphs, one per mask-word width.
+  struct GExec {
+    hipGraphExec_t exec = nullptr;
+    int K = 0;
+    int prof = -1;
+  };
+  GExec gx[4];
+  double clock_khz = 100000.0;  // device wall-clock rate (wall_clock64 ticks)
+  // Per-phase timing of init / WCC / output (launch timing on, outside graphs).
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  bool timed() const { return (opts.profile & 1) != 0; }
+  bool counted() const { return (opts.profile & 2) != 0; }
+  // Continuous-batching session (serve.hip), null outside serving.
+  ServeState* serve = nullptr;
+
+  KP kp() const {
+    KP P{};
+    P.V = g->V;
+    P.E = g->E;
+    P.profile = (opts.profile & 2) ? 1 : 0;
+    P.off = g->off;
+    P.idx = g->idx;
+    P.wt = g->weighted ? g->wt : nullptr;
+    P.rows = g->rows;
+    P.visited = (u64*)visited.p;
+    P.next = (u64*)next.p;
+    P.front = (u64*)front.p;
+    P.mark = (u32*)mark.p;
+    P.vals = (u32*)vals.p;
+    P.touched = (int32_t*)touched.p;
+    for (int k = 0; k < 2; ++k) {
+      P.small[k] = (int32_t*)small[k].p;
+      P.medium[k] = (int32_t*)medium[k].p;
+      P.large[k] = (u64*)large[k].p;
+      P.dense[k] = (u64*)dense[k].p;
+    }
+    P.ctl = d_ctl;
+    P.cfg = d_cfg;
+    P.bin_small = opts.bin_small;
+    P.bin_large = opts.bin_large;
+    return P;
+  }
+
+  void drop_graphs() {
+    for (int k = 0; k < 4; ++k) {
+      if (gx[k].exec) (void)hipGraphExecDestroy(gx[k].exec);
+      gx[k].exec = nullptr;
+      gx[k].K = 0;
+      gx[k].prof = -1;
+    }
+  }
+
+  // Grid sizes: persistent-style, sized to the device, independent of the
+  // frontier so the launches can be captured once and replayed.
+  // Same thread count per launch for every workgroup size.
+  int grid_advance() const { return cus * 8 * 256 / std::max(128, opts.block_size); }
+  int grid_filter() const {
+    int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));
+  }
+
+  void ensure_words(int W) {
+    const size_t V = (size_t)g->V, E = (size_t)g->E;
+    bool moved = false;
+    if (W > wcap) {
+      moved |= visited.ensure(V * W * 8);
+      moved |= next.ensure(V * W * 8);
+      moved |= front.ensure(V * W * 8);
+      HIP_CHECK(hipMemsetAsync(next.p, 0, V * W * 8, s));
+      for (int k = 0; k < 2; ++k) moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
+      wcap = W;
+    }
+    if (!mark.p) {
+      moved |= mark.ensure(V * 4 + 4);
+      HIP_CHECK(hipMemsetAsync(mark.p, 0, V * 4 + 4, s));
+      moved |= touched.ensure(V * 4 + 4);
+      for (int k = 0; k < 2; ++k) {
+        moved |= small[k].ensure(V * 4 + 4);
+        moved |= medium[k].ensure(V * 4 + 4);
+        moved |= large[k].ensure((V + E / LCHUNK + 1) * 8);
+      }
+    }
+    if (moved) drop_graphs();
+  }
+
+  void ensure_vals(size_t elems) {
+    if (vals.ensure(elems * 4)) drop_graphs();
+  }
+
+  void stage(size_t ints) {
+    if (ints <= h_stage_cap) return;
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    HIP_CHECK(hipHostMalloc((void**)&h_stage, ints * 4, hipHostMallocDefault));
+    h_stage_cap = ints;
+  }
+
+  // ---- timing helpers (launch timing on)
+  void tick(int cat, bool begin) {
+    if (!timed()) return;
+    if (ev_used >= ev_pool.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    hipEvent_t e = ev_pool[ev_used++];
+    HIP_CHECK(hipEventRecord(e, s));
+    if (begin) ev.push_back({cat, {e, nullptr}});
+    else ev.back().second.second = e;
+  }
+};
+
+
+namespace cgpi {
+
+enum { CAT_INIT = 0, CAT_ADV = 1, CAT_FILTER = 2, CAT_WCC = 3, CAT_OUT = 4 };
+
+// Launchers implemented in engine.hip (W in {1, 2, 4, 8}).
+void dispatch_iteration(cgp_engine* en, int W, const KP& P);
+void dispatch_seed_only(cgp_engine* en, int W, const KP& P, const int32_t* d_slot_src, int nslots);
+void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* d_slot_src, int nslots);
+void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus);
+// One batched run (all passes, WCC, output transpose); stats may be null.
+void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out, bool out_dev,
+                      cgp_run_stats* stats);
+// Computes min-id component labels into en->parent (hooking + pointer jumping).
+void compute_wcc(cgp_engine* en);
+// Writes the labels into rows rows_out[0..nrows) of out (query-major, V per row).
+void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out);
+
+}  // namespace cgpi

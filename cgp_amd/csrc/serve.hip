@@ -1,12 +1,408 @@
-// serve.hip — placeholder, replaced by the continuous-batching implementation.
-#include "cgp_internal.hpp"
-extern "C" {
-cgp_status cgp_serve_start(cgp_engine*, const cgp_serve_opts*) { cgpi::set_last_error("serving: not built"); return CGP_EINVAL; }
-cgp_status cgp_serve_submit(cgp_engine*, const cgp_query*, int64_t, double) { return CGP_EINVAL; }
-cgp_status cgp_serve_step(cgp_engine*, int32_t, cgp_completion*, int32_t, int32_t*) { return CGP_EINVAL; }
-double cgp_serve_clock(const cgp_engine*) { return 0.0; }
-cgp_status cgp_serve_pending(const cgp_engine*, int32_t*, int32_t*) { return CGP_EINVAL; }
-void cgp_serve_release_result(uint32_t*) {}
-cgp_status cgp_serve_stop(cgp_engine*) { return CGP_EINVAL; }
-cgp_status cgp_engine_autotune(cgp_engine*, const cgp_query*, int32_t, cgp_engine_opts*) { return CGP_EINVAL; }
+// serve.hip — continuous batching: queries arriving while others run are
+// admitted into free bitmask slots at iteration boundaries, all active slots
+// traverse together in every iteration, and a slot's result is snapshotted
+// and the slot recycled as soon as its query converges.
+//
+// Replaces run_replenish_pipeline / replenish_frontier_engine::run
+// (puercgp/include/puercgp/engine/replenish_engine.hxx:41-410, 559-573) and
+// its slot kernels (hybrid_engine.hxx:563-784).  Differences by design:
+//   * per-query state stays on the device; admission is a seed kernel that
+//     joins the next filter, with no per-iteration descriptor re-upload;
+//   * BFS depths stay relative to the query's own start (the slot's start
+//     iteration), so a query admitted at iteration L gets the same depths as
+//     alone -- the reference writes the global level (hybrid_engine.hxx:172-175);
+//   * admission is capped by an HBM budget for the per-query value rows
+//     (V * 4 bytes each): arrivals queue, they never over-allocate;
+//   * WCC requests are source-free: they share one hooking/pointer-jumping
+//     result per session and complete at admission.
+#include <deque>
+
+#include "engine_impl.hpp"
+
+using namespace cgpk;
+using namespace cgpi;
+
+namespace {
+
+// For each listed slot (word, lane) -> output row: copies the slot's value
+// column into out[row * V + v], then resets the column to INF and clears the
+// slot's visited bit, leaving the slot ready for its next query.
+__global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visited, int32_t V, int W,
+                                                          int rshift, const int32_t* slot_word,
+                                                          const int32_t* slot_lane, const int32_t* slot_row,
+                                                          int nslots, const u64* clear_mask, u32* out) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    for (int k = 0; k < nslots; ++k) {
+      const size_t at = (((size_t)slot_word[k] * (size_t)V + (size_t)v) << rshift) + (size_t)slot_lane[k];
+      if (out) out[(size_t)slot_row[k] * V + v] = vals[at];
+      vals[at] = INF;
+    }
+    for (int w = 0; w < W; ++w)
+      if (clear_mask[w]) visited[(size_t)v * W + w] &= ~clear_mask[w];
+  }
 }
+
+struct Pending {
+  cgp_query q;
+  int64_t ticket;
+  double arrival;
+};
+
+struct SlotState {
+  bool busy = false;
+  cgp_query q{};
+  int64_t ticket = 0;
+  double arrival = 0, admit = 0;
+  u32 start = 0;
+};
+
+}  // namespace
+
+struct ServeState {
+  int W = 1;               // mask words (rounded)
+  int nslots = 64;         // W * 64 slots exist
+  int budget_slots = 64;   // slots the HBM budget allows to be busy at once
+  bool keep = true;
+  std::chrono::steady_clock::time_point t0;
+  std::deque<Pending> queue;
+  std::vector<SlotState> slots;
+  int busy = 0;
+  u32 iter = 0;            // device iteration counter mirror
+  bool need_restart = false;
+  std::deque<cgp_completion> done;
+  // WCC labels of the session (computed on the first WCC request).
+  bool wcc_ready = false;
+  std::vector<u32> wcc_host;
+  // Device scratch for snapshots.
+  cgpi::DevBuf d_word, d_lane, d_row, d_mask, d_rows_out, d_src;
+  // Host result buffers (pinned) recycled after cgp_serve_release_result.
+  std::vector<uint32_t*> pool;
+  double clock() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+std::mutex g_pool_mu;
+// Result buffers handed out to callers, with their owning session's pool.
+std::vector<std::pair<uint32_t*, ServeState*>> g_out;
+
+uint32_t* take_buffer(ServeState* ss, size_t V) {
+  if (!ss->pool.empty()) {
+    uint32_t* b = ss->pool.back();
+    ss->pool.pop_back();
+    return b;
+  }
+  uint32_t* b = nullptr;
+  HIP_CHECK(hipHostMalloc((void**)&b, V * 4 + 4, hipHostMallocDefault));
+  return b;
+}
+
+// Admits queued queries into free slots (within the budget) and seeds them.
+void admit(cgp_engine* en, ServeState* ss) {
+  const int32_t V = en->g->V;
+  Cfg& cfg = *en->h_cfg;
+  int32_t* h_src = en->h_stage;
+  bool any = false;
+  for (int k = 0; k < ss->nslots; ++k) h_src[k] = -1;
+  while (!ss->queue.empty()) {
+    Pending pd = ss->queue.front();
+    if (pd.q.kind == CGP_WCC) {
+      ss->queue.pop_front();
+      if (!ss->wcc_ready) {
+        compute_wcc(en);
+        ss->wcc_host.resize((size_t)V);
+        HIP_CHECK(hipMemcpyAsync(ss->wcc_host.data(), en->parent.p, (size_t)V * 4, hipMemcpyDeviceToHost, en->s));
+        HIP_CHECK(hipStreamSynchronize(en->s));
+        ss->wcc_ready = true;
+      }
+      cgp_completion c{};
+      c.ticket = pd.ticket;
+      c.kind = CGP_WCC;
+      c.source = pd.q.source;
+      c.arrival_s = pd.arrival;
+      c.admit_s = ss->clock();
+      c.iterations = 0;
+      if (ss->keep) {
+        c.result = take_buffer(ss, (size_t)V);
+        std::memcpy(c.result, ss->wcc_host.data(), (size_t)V * 4);
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        g_out.push_back({c.result, ss});
+      }
+      c.done_s = ss->clock();
+      ss->done.push_back(c);
+      continue;
+    }
+    if (ss->busy >= ss->budget_slots) break;
+    int slot = -1;
+    for (int k = 0; k < ss->nslots; ++k)
+      if (!ss->slots[k].busy) {
+        slot = k;
+        break;
+      }
+    if (slot < 0) break;
+    ss->queue.pop_front();
+    SlotState& st = ss->slots[slot];
+    st.busy = true;
+    st.q = pd.q;
+    st.ticket = pd.ticket;
+    st.arrival = pd.arrival;
+    st.admit = ss->clock();
+    st.start = ss->iter + 1;  // the seed is settled by this iteration's filter
+    const int w = slot >> 6, q = slot & 63;
+    const u64 bit = 1ull << q;
+    cfg.bfs_bits[w] &= ~bit;
+    cfg.val_bits[w] &= ~bit;
+    if (pd.q.kind == CGP_BFS) cfg.bfs_bits[w] |= bit;
+    else cfg.val_bits[w] |= bit;
+    cfg.start[slot] = (int32_t)st.start;
+    h_src[slot] = pd.q.source;
+    ++ss->busy;
+    any = true;
+  }
+  if (!any) return;
+  HIP_CHECK(hipMemcpyAsync(en->d_cfg, en->h_cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
+  HIP_CHECK(hipMemcpyAsync(ss->d_src.p, h_src, (size_t)ss->nslots * 4, hipMemcpyHostToDevice, en->s));
+  if (ss->need_restart) {
+    HIP_CHECK(hipMemsetAsync(&en->d_ctl->finished, 0, 4, en->s));
+    ss->need_restart = false;
+  }
+  dispatch_seed_only(en, ss->W, en->kp(), (const int32_t*)ss->d_src.p, ss->nslots);
+}
+
+// Snapshots converged slots, recycles them and queues their completions.
+void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
+  const int32_t V = en->g->V;
+  const int k = (int)conv.size();
+  int32_t* h = en->h_stage;  // pinned: [word k][lane k][row k][pad][mask W]
+  u64* mask = reinterpret_cast<u64*>(h + ((3 * k + 1) & ~1));
+  for (int w = 0; w < ss->W; ++w) mask[w] = 0ull;
+  for (int i = 0; i < k; ++i) {
+    h[i] = conv[i] >> 6;
+    h[k + i] = conv[i] & 63;
+    h[2 * k + i] = i;
+    mask[conv[i] >> 6] |= 1ull << (conv[i] & 63);
+  }
+  ss->d_word.ensure((size_t)k * 4);
+  ss->d_lane.ensure((size_t)k * 4);
+  ss->d_row.ensure((size_t)k * 4);
+  ss->d_mask.ensure((size_t)ss->W * 8);
+  if (ss->keep) ss->d_rows_out.ensure((size_t)k * V * 4);
+  HIP_CHECK(hipMemcpyAsync(ss->d_word.p, h, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
+  HIP_CHECK(hipMemcpyAsync(ss->d_lane.p, h + k, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
+  HIP_CHECK(hipMemcpyAsync(ss->d_row.p, h + 2 * k, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
+  HIP_CHECK(hipMemcpyAsync(ss->d_mask.p, mask, (size_t)ss->W * 8, hipMemcpyHostToDevice, en->s));
+  hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
+                     (u64*)en->visited.p, V, ss->W, 6, (const int32_t*)ss->d_word.p,
+                     (const int32_t*)ss->d_lane.p, (const int32_t*)ss->d_row.p, k,
+                     (const u64*)ss->d_mask.p, ss->keep ? (u32*)ss->d_rows_out.p : nullptr);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint32_t*> bufs((size_t)k, nullptr);
+  if (ss->keep) {
+    for (int i = 0; i < k; ++i) {
+      bufs[i] = take_buffer(ss, (size_t)V);
+      HIP_CHECK(hipMemcpyAsync(bufs[i], (const u32*)ss->d_rows_out.p + (size_t)i * V, (size_t)V * 4,
+                               hipMemcpyDeviceToHost, en->s));
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(en->s));
+  const double now = ss->clock();
+  for (int i = 0; i < k; ++i) {
+    SlotState& st = ss->slots[conv[i]];
+    cgp_completion c{};
+    c.ticket = st.ticket;
+    c.kind = st.q.kind;
+    c.source = st.q.source;
+    c.arrival_s = st.arrival;
+    c.admit_s = st.admit;
+    c.done_s = now;
+    c.iterations = (int32_t)(ss->iter - st.start + 1);
+    c.result = bufs[i];
+    if (bufs[i]) {
+      std::lock_guard<std::mutex> lk(g_pool_mu);
+      g_out.push_back({bufs[i], ss});
+    }
+    ss->done.push_back(c);
+    st.busy = false;
+    --ss->busy;
+  }
+}
+
+void serve_free(cgp_engine* en) {
+  ServeState* ss = en->serve;
+  if (!ss) return;
+  (void)hipStreamSynchronize(en->s);
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    // Buffers still held by the caller stay valid; they are freed on release.
+    for (auto& o : g_out)
+      if (o.second == ss) o.second = nullptr;
+  }
+  for (uint32_t* b : ss->pool) (void)hipHostFree(b);
+  for (auto& c : ss->done)
+    if (c.result) cgp_serve_release_result(c.result);
+  ss->d_word.release();
+  ss->d_lane.release();
+  ss->d_row.release();
+  ss->d_mask.release();
+  ss->d_rows_out.release();
+  ss->d_src.release();
+  // Stamps used by the session must not recur in later runs; slots may hold
+  // bits in next[] when stopped mid-run.
+  en->stamp_epoch += ss->iter + 2;
+  en->next_dirty = true;
+  delete ss;
+  en->serve = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
+  CGP_TRY({
+    if (!en) fail(CGP_EINVAL, "cgp_serve_start: null engine");
+    std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
+    if (!lk.owns_lock() || en->serve) fail(CGP_EBUSY, "cgp_serve_start: engine is busy");
+    const int32_t V = en->g->V;
+    if (V <= 0) fail(CGP_EINVAL, "cgp_serve_start: empty graph");
+    const uint64_t per_query = (uint64_t)V * 4;
+    int budget = (int)std::min<uint64_t>((uint64_t)MAXW * 64, en->opts.hbm_budget_bytes / per_query);
+    if (budget < 1)
+      fail(CGP_ECAPACITY, "cgp_serve_start: HBM budget smaller than one query's value row (" +
+                              std::to_string(per_query) + " bytes)");
+    int slots = so && so->max_slots > 0 ? std::min(so->max_slots, MAXW * 64) : budget;
+    budget = std::min(budget, slots);
+    const int W = round_words((slots + 63) / 64);
+    std::unique_ptr<ServeState> ss(new ServeState());
+    ss->W = W;
+    ss->nslots = W * 64;
+    ss->budget_slots = budget;
+    ss->keep = !so || so->keep_results != 0;
+    ss->slots.resize((size_t)ss->nslots);
+    en->ensure_words(W);
+    en->ensure_vals((size_t)W * V * 64);
+    en->stage((size_t)ss->nslots * 4 + 64);
+    ss->d_src.ensure((size_t)ss->nslots * 4);
+    // Clean slot state: every value row INF, no visited bits, no pending next bits.
+    fill_u32(en->s, en->vals.p, (size_t)W * V * 64, INF, en->cus);
+    fill_u32(en->s, en->visited.p, (size_t)V * W * 2, 0u, en->cus);
+    if (en->next_dirty) {
+      fill_u32(en->s, en->next.p, (size_t)V * W * 2, 0u, en->cus);
+      en->next_dirty = false;
+    }
+    Cfg& cfg = *en->h_cfg;
+    std::memset(&cfg, 0, sizeof(Cfg));
+    cfg.rshift = 6;
+    cfg.W = W;
+    cfg.pull_policy = 0;
+    cfg.dense_t = en->opts.dense_threshold;
+    cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
+    Ctl& c = *en->h_ctl;
+    std::memset(&c, 0, sizeof(Ctl));
+    c.iter = 0;
+    if (en->stamp_epoch > 0xE0000000u) {
+      HIP_CHECK(hipMemsetAsync(en->mark.p, 0, (size_t)V * 4, en->s));
+      en->stamp_epoch = 1;
+    }
+    c.stamp_base = en->stamp_epoch;
+    c.adv_t0 = ~0ull;
+    c.flt_t0 = ~0ull;
+    HIP_CHECK(hipMemcpyAsync(en->d_cfg, &cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
+    HIP_CHECK(hipMemcpyAsync(en->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, en->s));
+    HIP_CHECK(hipStreamSynchronize(en->s));
+    ss->t0 = std::chrono::steady_clock::now();
+    en->serve = ss.release();
+  });
+}
+
+cgp_status cgp_serve_submit(cgp_engine* en, const cgp_query* q, int64_t ticket, double arrival_s) {
+  CGP_TRY({
+    if (!en || !q) fail(CGP_EINVAL, "cgp_serve_submit: null argument");
+    if (!en->serve) fail(CGP_EINVAL, "cgp_serve_submit: no serving session (call cgp_serve_start)");
+    if (q->kind != CGP_BFS && q->kind != CGP_SSSP && q->kind != CGP_WCC)
+      fail(CGP_EINVAL, "cgp_serve_submit: unknown query kind " + std::to_string(q->kind));
+    if (q->kind != CGP_WCC && (q->source < 0 || q->source >= en->g->V))
+      fail(CGP_EINVAL, "cgp_serve_submit: query source out of range: " + std::to_string(q->source));
+    en->serve->queue.push_back(Pending{*q, ticket, arrival_s});
+  });
+}
+
+cgp_status cgp_serve_step(cgp_engine* en, int32_t max_iterations, cgp_completion* out, int32_t cap,
+                          int32_t* n_done) {
+  CGP_TRY({
+    if (!en || !n_done || (cap > 0 && !out)) fail(CGP_EINVAL, "cgp_serve_step: null argument");
+    *n_done = 0;
+    ServeState* ss = en->serve;
+    if (!ss) fail(CGP_EINVAL, "cgp_serve_step: no serving session (call cgp_serve_start)");
+    const int W = ss->W;
+    int ran = 0;
+    std::vector<int> conv;
+    for (;;) {
+      admit(en, ss);
+      if (ss->busy == 0) break;  // idle: nothing admitted or running
+      if (max_iterations > 0 && ran >= max_iterations) break;
+      dispatch_iteration(en, W, en->kp());
+      HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
+      HIP_CHECK(hipStreamSynchronize(en->s));
+      ++ran;
+      const Ctl& hc = *en->h_ctl;
+      ss->iter = hc.iter;
+      // Converged: busy slots whose bit left the frontier, or every busy slot
+      // once the frontier has no work left (`finished`: its remaining bits sit
+      // on vertices without out-edges, and the kernels stop until restarted).
+      conv.clear();
+      for (int k = 0; k < ss->nslots; ++k) {
+        if (!ss->slots[k].busy || ss->iter < ss->slots[k].start) continue;
+        if (hc.finished || !((hc.last_union[k >> 6] >> (k & 63)) & 1ull)) conv.push_back(k);
+      }
+      if (hc.finished) ss->need_restart = true;
+      if (!conv.empty()) retire(en, ss, conv);
+      if (max_iterations == 0 && !ss->done.empty()) break;
+      // Every query converges within V + 1 iterations of its admission.
+      if (ran > en->g->V + 3 && conv.empty() && ss->done.empty())
+        fail(CGP_EDEVICE, "cgp_serve_step: no query converged within V+3 iterations");
+    }
+    int32_t nd = 0;
+    while (nd < cap && !ss->done.empty()) {
+      out[nd++] = ss->done.front();
+      ss->done.pop_front();
+    }
+    *n_done = nd;
+  });
+}
+
+double cgp_serve_clock(const cgp_engine* en) { return en && en->serve ? en->serve->clock() : 0.0; }
+
+cgp_status cgp_serve_pending(const cgp_engine* en, int32_t* queued, int32_t* running) {
+  CGP_TRY({
+    if (!en || !en->serve) fail(CGP_EINVAL, "cgp_serve_pending: no serving session");
+    if (queued) *queued = (int32_t)en->serve->queue.size();
+    if (running) *running = en->serve->busy;
+  });
+}
+
+void cgp_serve_release_result(uint32_t* r) {
+  if (!r) return;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (size_t i = 0; i < g_out.size(); ++i) {
+    if (g_out[i].first != r) continue;
+    ServeState* owner = g_out[i].second;
+    g_out.erase(g_out.begin() + (long)i);
+    if (owner) owner->pool.push_back(r);
+    else (void)hipHostFree(r);
+    return;
+  }
+}
+
+cgp_status cgp_serve_stop(cgp_engine* en) {
+  CGP_TRY({
+    if (!en) fail(CGP_EINVAL, "cgp_serve_stop: null engine");
+    if (!en->serve) fail(CGP_EINVAL, "cgp_serve_stop: no serving session");
+    serve_free(en);
+  });
+}
+
+
+}  // extern "C"

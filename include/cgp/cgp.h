@@ -75,7 +75,7 @@ typedef struct {
   double pull_edge_ratio;      /* pull when frontier edges >= ratio * E (default 0.20, types.hxx:31) */
   int32_t max_iterations;      /* 0 = until convergence (types.hxx:36) */
   int32_t max_words;           /* cap on 64-bit mask words per pass; 0 = tuned */
-  int32_t block_size;          /* workgroup size of the advance kernels (64..1024); 0 = tuned */
+  int32_t block_size;          /* workgroup size of the advance kernel: 128, 256 or 512; 0 = default 256 */
   int32_t bin_small;           /* degree <= bin_small: 8 lanes per vertex; 0 = tuned */
   int32_t bin_large;           /* degree > bin_large: one workgroup per vertex; 0 = tuned */
   int32_t use_hip_graph;       /* capture iterations into a hipGraph (default 1) */
@@ -83,6 +83,10 @@ typedef struct {
   int32_t profile;             /* bit 0: per-launch HIP events (also inside graph replays);
                                   bit 1: device work counters -> advance_bytes */
   uint64_t hbm_budget_bytes;   /* serving: cap on per-query value memory; 0 = 64 GiB */
+  int32_t dense_threshold;     /* active SSSP queries in a mask word at which the word is relaxed
+                                  lane-per-query (one wave per edge row); 0 = 16 */
+  int32_t frontier_dense_div;  /* dense frontier mode (fire-and-forget atomics, filter scans all
+                                  vertices) once the frontier's edges reach V / div; 0 = 8 */
 } cgp_engine_opts;
 
 /* Per-run report; replaces run_result_t / iteration_profile_t

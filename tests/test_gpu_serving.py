@@ -1,0 +1,175 @@
+"""GPU: continuous batching (admission at iteration boundaries, HBM budget,
+per-query results identical alone and in the mix) and the autotuner."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cgp_amd
+import oracle_lib as O
+from cgp_amd import BFS, SSSP, WCC, INF
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+
+
+def drain(en, want, max_steps=5000):
+    done = {}
+    steps = 0
+    while len(done) < want and steps < max_steps:
+        for c in en.serve_step(0):
+            done[c["ticket"]] = c
+        steps += 1
+    return done
+
+
+def test_replenish_golden_two_slots(gpu):
+    # validate_replenish.cu: 4 queries through 2 slots on T2; the late BFS
+    # query must get depths relative to its own start (the reference writes the
+    # global level there).
+    t = GOLD["graphs"]["T2"]
+    g = cgp_amd.Graph.from_csr(t["offsets"], t["indices"], t["weights"])
+    en = cgp_amd.Engine(g)
+    en.serve_start(max_slots=2)
+    for i, (k, s) in enumerate([(BFS, 0), (SSSP, 0), (BFS, 1), (SSSP, 1)]):
+        en.serve_submit(k, s, i, 0.0)
+    done = drain(en, 4)
+    en.serve_stop()
+    exp = [[0, 1, 1, 2], [0, 2, 5, 5], [INF, 0, INF, 1], [INF, 0, INF, 3]]
+    for i in range(4):
+        assert done[i]["result"].tolist() == exp[i]
+    assert max(done[2]["admit_s"], done[3]["admit_s"]) >= min(done[0]["done_s"], done[1]["done_s"])
+
+
+@pytest.mark.parametrize("slots", [4, 64, 130])
+def test_staggered_admission_matches_oracle(gpu, slots):
+    h = cgp_amd.HostCSR.rmat(12, 16, seed=21, wmin=1, wmax=200)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = h.sample_sources(90, seed=4)
+    kinds = [(BFS, SSSP, BFS)[i % 3] for i in range(90)]
+    en = cgp_amd.Engine(g)
+    en.serve_start(max_slots=slots)
+    done = {}
+    nxt = 0
+    steps = 0
+    submitted_wcc = set()
+    # Open-loop trickle: a few queries per step, so later ones join running ones.
+    while len(done) < 92:
+        steps += 1
+        assert steps < 20000, "serving made no progress"
+        for _ in range(3):
+            if nxt < 90:
+                en.serve_submit(kinds[nxt], int(src[nxt]), nxt, en.serve_clock())
+                nxt += 1
+        if nxt == 45 and 1000 not in submitted_wcc:
+            en.serve_submit(WCC, 0, 1000, en.serve_clock())
+            en.serve_submit(WCC, 0, 1001, en.serve_clock())
+            submitted_wcc.add(1000)
+        for c in en.serve_step(max_iterations=1):
+            done[c["ticket"]] = c
+        q, r = en.serve_pending()
+        assert r <= slots
+    en.serve_stop()
+    for i in range(90):
+        exp = O.expected(og, 0 if kinds[i] == BFS else 1, src[i])
+        assert np.array_equal(done[i]["result"], exp), i
+    lab = O.wcc_uf(og)
+    assert np.array_equal(done[1000]["result"], lab) and np.array_equal(done[1001]["result"], lab)
+    # Later queries were admitted while earlier ones were still running.
+    admits = sorted(c["admit_s"] for c in done.values())
+    finishes = sorted(c["done_s"] for c in done.values())
+    assert admits[-1] > finishes[0]
+
+
+def test_results_same_alone_and_in_the_mix(gpu):
+    h = cgp_amd.HostCSR.rmat(11, 16, seed=22, wmin=1, wmax=64)
+    g = cgp_amd.Graph.from_host(h)
+    src = h.sample_sources(24, seed=8)
+    kinds = [BFS, SSSP] * 12
+    en = cgp_amd.Engine(g)
+    alone = en.run_batch(kinds, src)[0]
+    en.serve_start(max_slots=7)
+    for i in range(24):
+        en.serve_submit(kinds[i], int(src[i]), i, 0.0)
+    done = drain(en, 24)
+    en.serve_stop()
+    for i in range(24):
+        assert np.array_equal(done[i]["result"], alone[i])
+
+
+def test_hbm_budget_caps_running_slots(gpu):
+    h = cgp_amd.HostCSR.rmat(10, 16, seed=23, wmin=1, wmax=9)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = h.sample_sources(20, seed=1)
+    en = cgp_amd.Engine(g, hbm_budget_bytes=3 * g.V * 4)
+    en.serve_start()
+    for i in range(20):
+        en.serve_submit(SSSP, int(src[i]), i, 0.0)
+    done = {}
+    steps = 0
+    while len(done) < 20:
+        steps += 1
+        assert steps < 20000, "serving made no progress"
+        for c in en.serve_step(max_iterations=1):
+            done[c["ticket"]] = c
+        assert en.serve_pending()[1] <= 3
+    en.serve_stop()
+    for i in range(20):
+        assert np.array_equal(done[i]["result"], O.sssp(og, src[i]))
+    small = cgp_amd.Engine(g, hbm_budget_bytes=g.V * 4 - 1)
+    with pytest.raises(cgp_amd.CgpError) as e:
+        small.serve_start()
+    assert e.value.status == cgp_amd.CGP_ECAPACITY
+
+
+def test_serving_errors_and_busy(gpu):
+    h = cgp_amd.HostCSR.rmat(8, 8, seed=2)
+    g = cgp_amd.Graph.from_host(h)
+    en = cgp_amd.Engine(g)
+    with pytest.raises(cgp_amd.CgpError):
+        en.serve_submit(BFS, 0, 0, 0.0)  # no session
+    en.serve_start(max_slots=4, keep_results=0)
+    with pytest.raises(cgp_amd.CgpError) as e:
+        en.run_batch([BFS], [0])
+    assert e.value.status == cgp_amd.CGP_EBUSY
+    with pytest.raises(cgp_amd.CgpError):
+        en.serve_submit(BFS, g.V, 0, 0.0)
+    en.serve_submit(BFS, int(h.sample_sources(1, 1)[0]), 5, 0.0)
+    done = drain(en, 1)
+    assert done[5]["result"] is None and done[5]["done_s"] >= done[5]["admit_s"]
+    en.serve_stop()
+    out = en.run_batch([BFS], [0])[0]  # batch runs work again after stop
+    assert out.shape == (1, g.V)
+
+
+@pytest.mark.parametrize("bs", [128, 512])
+def test_block_sizes(gpu, bs):
+    h = cgp_amd.HostCSR.rmat(12, 16, seed=24, wmin=1, wmax=100)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = h.sample_sources(100, seed=9)
+    kinds = [BFS] * 60 + [SSSP] * 40
+    out = cgp_amd.Engine(g, block_size=bs).run_batch(kinds, src)[0]
+    for i in range(0, 100, 11):
+        assert np.array_equal(out[i], O.expected(og, 0 if kinds[i] == BFS else 1, src[i]))
+
+
+def test_autotune_then_parity(gpu):
+    h = cgp_amd.HostCSR.rmat(12, 16, seed=25, wmin=1, wmax=255)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = h.sample_sources(140, seed=3)
+    kinds = [BFS] * 80 + [SSSP] * 60
+    en = cgp_amd.Engine(g)
+    chosen = en.autotune(kinds, src)
+    assert chosen["block_size"] in (128, 256, 512)
+    assert 1 <= chosen["max_words"] <= 8
+    assert en.opts["bin_small"] == chosen["bin_small"]
+    src2 = h.sample_sources(140, seed=77)
+    out = en.run_batch(kinds, src2)[0]
+    for i in range(0, 140, 9):
+        assert np.array_equal(out[i], O.expected(og, 0 if kinds[i] == BFS else 1, src2[i]))
