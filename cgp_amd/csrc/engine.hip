@@ -127,40 +127,58 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
                                            u64& nimp) {
   bool first = false;
   const size_t vb = (size_t)v * W;
+  // All W visited / next words of v in one go (each array's W words are one
+  // contiguous sector), issued before any of them is used.
+  u64 nx[W], vis[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    nx[w] = P.next[vb + w];
+    vis[w] = P.visited[vb + w];
+  }
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     if (!(bm[w] | sm[w])) continue;
-    u64* pn = &P.next[vb + w];
     u64 add = 0;
-    const u64 nx = *pn;
     if (bm[w]) {
       ++nbfs;
-      add = bm[w] & ~P.visited[vb + w];
+      add = bm[w] & ~vis[w];
     }
-    if (sm[w]) {
-      u64 s = sm[w];
+    u64 s = sm[w];
+    if (s) {
       const size_t ub = vrow(P.V, w, u, rshift), vr = vrow(P.V, w, v, rshift);
       while (s) {
-        const int q = __ffsll((long long)s) - 1;
-        s &= s - 1;
-        ++nsssp;
-        const u32 du = P.vals[ub + q];
-        const u32 cand = du + wv;
-        if (cand < du) continue;  // saturate: never wrap past CGP_INF
-        u32* pv = &P.vals[vr + q];
-        if (cand < *pv) {
-          atomicMin(pv, cand);
-          add |= 1ull << q;
-          ++nimp;
+        // Up to 4 active queries per round, their 8 value loads in flight together.
+        int q[4];
+        bool a[4];
+        u32 du[4], cur[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a[i] = s != 0;
+          q[i] = a[i] ? __ffsll((long long)s) - 1 : 0;
+          if (a[i]) s &= s - 1;
+          du[i] = P.vals[ub + q[i]];
+          cur[i] = P.vals[vr + q[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (!a[i]) continue;
+          ++nsssp;
+          const u32 cand = du[i] + wv;
+          if (cand < du[i]) continue;  // saturate: never wrap past CGP_INF
+          if (cand < cur[i]) {
+            atomicMin(&P.vals[vr + q[i]], cand);
+            add |= 1ull << q[i];
+            ++nimp;
+          }
         }
       }
     }
-    if (add & ~nx) {
+    if (add & ~nx[w]) {
       ++nnew;
       if (SPARSE) {
-        if (atomicOr(pn, add) == 0) first = true;
+        if (atomicOr(&P.next[vb + w], add) == 0) first = true;
       } else {
-        atomicOr(pn, add);
+        atomicOr(&P.next[vb + w], add);
       }
     }
   }
@@ -297,9 +315,11 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
 }
 
 // Phase D: a wave takes up to 64 dense (vertex, word, chunk) items at once,
-// flattens their edges as above, and walks them 8 per step with lane q as
-// query q: each step keeps 16 value rows in flight (8 source rows, mostly
-// cache hits, and 8 target rows), so no per-item latency chain remains.
+// flattens their edges as above, and walks them U per step with lane q as
+// query q.  Every value load is unconditional (inactive lanes read a clamped
+// address and discard it) so the U source rows and U target rows of a step
+// are all in flight together, and the loads of step k+1 are issued before
+// the atomics of step k, so waiting for them never waits on the atomics.
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
                                                   Tally& t) {
@@ -323,6 +343,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   const u32 incl = wave_incl_scan(d);
   const u32 total = (u32)__shfl((int)incl, 63, 64);
   const u32 smlo = (u32)sm, smhi = (u32)(sm >> 32);
+  const bool inrow = lane < R;
   for (u32 win = 0; win < total; win += 64) {
     const u32 f = win + (u32)lane;
     const int j = wave_upper_bound(incl, f);
@@ -331,47 +352,72 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
     const bool ok = f < total;
     const int32_t myv = ok ? P.idx[ej] : 0;
     const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : 0u;
+    const int myword = __shfl(w, j, 64);
     const int cnt = (int)min(64u, total - win);
-    for (int k0 = 0; k0 < cnt; k0 += U) {
-      int32_t vv[U], ww[U];
-      u32 cand[U], cur[U];
-      bool act[U];
+    // Per-step state: target row (w*V + v, scaled by R at use), weight,
+    // loaded values, activity.
+    u32 va[U];
+    u32 cand[U], cur[U], du[U];
+    bool act[U];
+    auto issue = [&](int k0) {
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         const int jj = min(k0 + k, cnt - 1);
-        vv[k] = __builtin_amdgcn_readlane(myv, jj);
-        const u32 wk = (u32)__builtin_amdgcn_readlane((int)myw, jj);
+        const int32_t vk = __builtin_amdgcn_readlane(myv, jj);
         const int it = __builtin_amdgcn_readlane(j, jj);
         const int32_t uk = __builtin_amdgcn_readlane(u, it);
-        ww[k] = __builtin_amdgcn_readlane(w, it);
+        const int wk = __builtin_amdgcn_readlane(w, it);
         const u64 smk = ((u64)(u32)__builtin_amdgcn_readlane((int)smhi, it) << 32) |
                         (u32)__builtin_amdgcn_readlane((int)smlo, it);
-        act[k] = (k0 + k < cnt) && lane < R && ((smk >> lane) & 1ull);
-        const u32 du = act[k] ? P.vals[vrow(P.V, ww[k], uk, rshift) + lane] : INF;
-        cand[k] = du + wk;
-        if (cand[k] < du) cand[k] = INF;
-        cur[k] = act[k] ? P.vals[vrow(P.V, ww[k], vv[k], rshift) + lane] : 0u;
+        act[k] = (k0 + k < cnt) && inrow && ((smk >> lane) & 1ull);
+        va[k] = act[k] ? (u32)wk * (u32)P.V + (u32)vk : 0u;
+        const u32 ur = act[k] ? (u32)wk * (u32)P.V + (u32)uk : 0u;
+        cand[k] = (u32)__builtin_amdgcn_readlane((int)myw, jj);
+        du[k] = P.vals[((size_t)ur << rshift) + (act[k] ? lane : 0)];
+        cur[k] = P.vals[((size_t)va[k] << rshift) + (act[k] ? lane : 0)];
       }
+    };
+    issue(0);
+    for (int k0 = 0; k0 < cnt; k0 += U) {
+      u32 va_c[U];
+      u32 cand_c[U];
+      bool imp[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        const bool imp = act[k] && cand[k] < cur[k];
-        if (imp) atomicMin(&P.vals[vrow(P.V, ww[k], vv[k], rshift) + lane], cand[k]);
-        const u64 m = __ballot(imp);
-        if (m) {
-          t.imp += imp ? 1 : 0;
+        u32 c = du[k] + cand[k];
+        if (c < du[k]) c = INF;
+        imp[k] = act[k] && c < cur[k];
+        cand_c[k] = c;
+        va_c[k] = va[k];
+      }
+      int32_t vv_c[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) vv_c[k] = __builtin_amdgcn_readlane(myv, min(k0 + k, cnt - 1));
+      if (k0 + U < cnt) issue(k0 + U);
+      u64 mine = 0;
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (imp[k]) atomicMin(&P.vals[((size_t)va_c[k] << rshift) + lane], cand_c[k]);
+        const u64 m = __ballot(imp[k]);
+        if (!SPARSE) {
+          mine = (lane == ((k0 + k) & 63)) ? m : mine;
+        } else if (m) {
           bool claimed = false;
           if (lane == 0) {
-            ++t.nnew;
-            if (SPARSE) {
-              if (atomicOr(&P.next[(size_t)vv[k] * W + ww[k]], m) == 0)
-                claimed = claim_vertex(P.mark, vv[k], stamp);
-            } else {
-              atomicOr(&P.next[(size_t)vv[k] * W + ww[k]], m);
-            }
+            const int wk = __builtin_amdgcn_readlane(myword, min(k0 + k, cnt - 1));
+            if (atomicOr(&P.next[(size_t)vv_c[k] * W + wk], m) == 0)
+              claimed = claim_vertex(P.mark, vv_c[k], stamp);
           }
-          if (SPARSE) wave_append<int32_t>(claimed, vv[k], &P.ctl->tcount, P.touched);
+          wave_append<int32_t>(claimed, vv_c[k], &P.ctl->tcount, P.touched);
+        }
+        if (m) {
+          t.imp += imp[k] ? 1 : 0;
+          if (lane == 0) ++t.nnew;
         }
       }
+      // Dense mode: lane k0+k owns edge k0+k's next-word update, one masked
+      // instruction for the whole step.
+      if (!SPARSE && mine) atomicOr(&P.next[(size_t)myv * W + myword], mine);
     }
   }
 }
