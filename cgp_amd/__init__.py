@@ -25,7 +25,8 @@ import numpy as np
 
 from ._lib import (BFS, CGP_EBUSY, CGP_ECAPACITY, CGP_EDEVICE, CGP_EINVAL, CGP_EIO, CGP_ENOMEM,
                    CGP_OK, HYBRID, INF, PULL, PUSH, SSSP, WCC, CgpError, cgp_completion,
-                   cgp_engine_opts, cgp_query, cgp_run_stats, cgp_serve_opts, check, lib)
+                   cgp_engine_opts, cgp_iteration_record, cgp_query, cgp_run_stats, cgp_serve_opts,
+                   check, lib)
 
 __all__ = [
     "BFS", "SSSP", "WCC", "PUSH", "PULL", "HYBRID", "INF", "CgpError", "HostCSR", "Graph",
@@ -267,6 +268,16 @@ class Engine:
             raise ValueError("device output buffer too small")
         check(lib.cgp_run_batch(self._h, q, n, out.ptr, 1, C.byref(st)))
         return None, st.as_dict()
+
+    def last_run_report(self):
+        """Per-iteration records of the last run_batch (list of dicts)."""
+        n = C.c_int32(0)
+        check(lib.cgp_last_run_report(self._h, None, 0, C.byref(n)))
+        if n.value == 0:
+            return []
+        recs = (cgp_iteration_record * n.value)()
+        check(lib.cgp_last_run_report(self._h, recs, n.value, C.byref(n)))
+        return [recs[i].as_dict() for i in range(n.value)]
 
     def bfs(self, sources):
         return self.run_batch([BFS] * len(sources), sources)[0]

@@ -70,6 +70,27 @@ class cgp_run_stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class cgp_iteration_record(C.Structure):
+    _fields_ = [
+        ("pass_", C.c_int32),
+        ("iteration", C.c_int32),
+        ("pull", C.c_int32),
+        ("dense", C.c_int32),
+        ("frontier_items", C.c_int64),
+        ("touched", C.c_int64),
+        ("next_items", C.c_int64),
+        ("next_edges", C.c_int64),
+        ("next_bfs_pairs", C.c_int64),
+        ("advance_ms", C.c_double),
+        ("filter_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["pass"] = d.pop("pass_")
+        return d
+
+
 class cgp_serve_opts(C.Structure):
     _fields_ = [("max_slots", C.c_int32), ("keep_results", C.c_int32)]
 
@@ -129,6 +150,7 @@ SIGNATURES = {
     "cgp_engine_get_opts": (_st, [_p, C.POINTER(cgp_engine_opts)]),
     "cgp_run_batch": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.c_int32, C.POINTER(cgp_run_stats)]),
     "cgp_reached_edges": (_st, [_p, _p, C.c_int32, _u64p]),
+    "cgp_last_run_report": (_st, [_p, C.POINTER(cgp_iteration_record), C.c_int32, _i32p]),
     "cgp_engine_autotune": (_st, [_p, C.POINTER(cgp_query), C.c_int32, C.POINTER(cgp_engine_opts)]),
     "cgp_serve_start": (_st, [_p, C.POINTER(cgp_serve_opts)]),
     "cgp_serve_submit": (_st, [_p, C.POINTER(cgp_query), C.c_int64, C.c_double]),

@@ -471,6 +471,60 @@ __device__ __forceinline__ bool pull_vertex(const KP& P, int32_t v, const u64 (&
   return SPARSE && first && claim_vertex(P.mark, v, stamp);
 }
 
+// Bottom-up step for a high-degree vertex v with the whole wave: each step
+// ORs the visited words of 64 neighbours, and the scan stops once every
+// missing bit was found.  Returns (on every lane) whether the wave made the
+// vertex's first claim of the iteration (sparse mode).
+template <int W, bool SPARSE>
+__device__ __forceinline__ bool pull_vertex_wave(const KP& P, int32_t v, const u64 (&bb)[W], u32 stamp,
+                                                 Tally& t) {
+  const int lane = lane_id();
+  u64 tgt[W], acc[W];
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    tgt[w] = bb[w] & ~P.visited[(size_t)v * W + w];
+    acc[w] = 0;
+    any |= tgt[w] != 0;
+  }
+  if (!any) return false;
+  if (lane == 0) ++t.pverts;
+  const int32_t e0 = P.off[v], e1 = P.off[v + 1];
+  for (int32_t eb = e0; eb < e1; eb += 64) {
+    const int32_t e = eb + lane;
+    const bool ok = e < e1;
+    const int32_t u = ok ? P.idx[e] : 0;
+    bool done = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const u64 a = ok ? (P.visited[(size_t)u * W + w] & tgt[w]) : 0ull;
+      acc[w] |= wave_or64(a);
+      done &= acc[w] == tgt[w];
+    }
+    if (ok) {
+      ++t.pedges;
+      t.pwords += W;
+    }
+    if (done) break;
+  }
+  bool first = false;
+  if (lane == 0) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (acc[w]) {
+        ++t.nnew;
+        if (SPARSE) {
+          if (atomicOr(&P.next[(size_t)v * W + w], acc[w]) == 0) first = true;
+        } else {
+          atomicOr(&P.next[(size_t)v * W + w], acc[w]);
+        }
+      }
+    }
+    first = SPARSE && first && claim_vertex(P.mark, v, stamp);
+  }
+  return first;
+}
+
 __device__ __forceinline__ u32 wave_fetch(u32* ctr, u32 n) {
   u32 i = 0;
   if (lane_id() == 0) i = atomicAdd(ctr, n);
@@ -531,14 +585,26 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
     if (base >= nS) break;
     expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, t);
   }
-  // Phase P.
+  // Phase P: high-degree vertices one wave each (64 neighbours per step,
+  // early exit on a wave-wide OR), then the rest one lane each.
   if (!bfs_on) {
     u64 bb[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
-    for (int64_t base = (int64_t)blockIdx.x * BS; base < P.V; base += (int64_t)gridDim.x * BS) {
-      const int32_t v = (int32_t)(base + threadIdx.x);
-      const bool claimed = v < P.V && pull_vertex<W, SPARSE>(P, v, bb, stamp, t);
+    const u32 nH = (u32)P.n_pull_hi;
+    while (nH) {
+      const u32 i = wave_fetch(&C->fetch[p][4], 1u);
+      if (i >= nH) break;
+      const int32_t v = P.pull_hi[i];
+      bool claimed = false;
+      if (pull_vertex_wave<W, SPARSE>(P, v, bb, stamp, t) && lane == 0) claimed = true;
+      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+    }
+    const int64_t nLo = P.n_pull_lo;
+    for (int64_t base = (int64_t)blockIdx.x * BS; base < nLo; base += (int64_t)gridDim.x * BS) {
+      const int64_t i = base + threadIdx.x;
+      const int32_t v = i < nLo ? P.pull_lo[i] : 0;
+      const bool claimed = i < nLo && pull_vertex<W, SPARSE>(P, v, bb, stamp, t);
       if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
     }
   }
@@ -735,12 +801,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     for (int w = 0; w < MAXW; ++w) {
       C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
     }
+    const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3];
     C->tcount = 0;
     C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = 0;
-    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = 0;
+    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = C->fetch[p][4] = 0;
     C->bfs_pairs[p] = 0;
     C->bfs_edges[p] = 0;
     C->done_blocks = 0;
+    const u32 cur_pull = C->bfs_pull;
     u32 pull = 0;
     if (fp > 0) {
       if (cfg->pull_policy == 1) pull = 1;
@@ -750,11 +818,26 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       }
     }
     C->bfs_pull = pull;
-    // Launch timing: the advance of this iteration ran iff iter != seeding.
+    // Launch timing and the iteration record: the advance of this iteration
+    // ran iff iter != seeding.
     if (iter != 0xFFFFFFFFu) {
-      C->adv_ticks += C->adv_t1 - C->adv_t0;
+      const u64 at = C->adv_t1 - C->adv_t0, ft = (u64)wall_clock64() - C->flt_t0;
+      C->adv_ticks += at;
       C->adv_launches += 1;
-      C->flt_ticks += (u64)wall_clock64() - C->flt_t0;
+      C->flt_ticks += ft;
+      if (P.iters) {
+        IterRec& r = P.iters[iter % ITER_RING];
+        r.iter = iter;
+        r.pull = cur_pull;
+        r.dense = scan ? 1u : 0u;
+        r.frontier_items = cur_items;
+        r.touched = n;
+        r.next_items = total;
+        r.next_edges = fall;
+        r.next_bfs_pairs = fp;
+        r.adv_ticks = at;
+        r.flt_ticks = ft;
+      }
     }
     C->adv_t0 = ~0ull;
     C->adv_t1 = 0;
@@ -1056,6 +1139,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     (q.kind == CGP_BFS ? bfs : q.kind == CGP_SSSP ? sssp : wcc).push_back(i);
   }
   cgp_run_stats st{};
+  std::vector<IterRec>& report = en->report;
+  report.clear();
   if (V == 0) {
     if (stats) *stats = st;
     return;
@@ -1162,6 +1247,21 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     st.pull_iterations += (int32_t)hc.pull_iters;
     st.passes += 1;
     if (en->counted()) add_work_bytes(st, hc.prof, W);
+    {
+      // Per-iteration records of this pass (the ring keeps the last ITER_RING).
+      const size_t nrec = std::min<size_t>(hc.iter, (size_t)ITER_RING);
+      const size_t first = report.size();
+      report.resize(first + nrec);
+      if (nrec) {
+        std::vector<IterRec> ring(nrec);
+        HIP_CHECK(hipMemcpy(ring.data(), en->iters.p, nrec * sizeof(IterRec), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nrec; ++k) {
+          const u32 it = hc.iter - (u32)nrec + (u32)k;
+          report[first + k] = ring[it % ITER_RING];
+          report[first + k].pull |= (u32)(st.passes - 1) << 16;  // pass index in the high half
+        }
+      }
+    }
     if (en->timed()) {
       st.advance_ms += (double)hc.adv_ticks / en->clock_khz;
       st.filter_ms += (double)hc.flt_ticks / en->clock_khz;
@@ -1287,6 +1387,21 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
     HIP_CHECK(hipMemcpy(g->off, off, ((size_t)V + 1) * 4, hipMemcpyHostToDevice));
     if (E) HIP_CHECK(hipMemcpy(g->idx, idx, (size_t)E * 4, hipMemcpyHostToDevice));
     if (wt && E) HIP_CHECK(hipMemcpy(g->wt, wt, (size_t)E * 4, hipMemcpyHostToDevice));
+    {
+      std::vector<int32_t> lo;
+      std::vector<int32_t> hi;
+      for (int32_t v = 0; v < V; ++v) {
+        const int32_t d = off[v + 1] - off[v];
+        if (d > PULL_WAVE_DEG) hi.push_back(v);
+        else if (d > 0) lo.push_back(v);
+      }
+      g->n_pull_lo = (int32_t)lo.size();
+      g->n_pull_hi = (int32_t)hi.size();
+      HIP_CHECK(hipMalloc((void**)&g->pull_lo, lo.size() * 4 + 4));
+      HIP_CHECK(hipMalloc((void**)&g->pull_hi, hi.size() * 4 + 4));
+      if (!lo.empty()) HIP_CHECK(hipMemcpy(g->pull_lo, lo.data(), lo.size() * 4, hipMemcpyHostToDevice));
+      if (!hi.empty()) HIP_CHECK(hipMemcpy(g->pull_hi, hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+    }
     if (V > 0) {
       int cus = device_cus();
       hipLaunchKernelGGL(k_fill_rows, dim3(cus * 16), dim3(BLOCK), 0, 0, (const int32_t*)g->off, V, g->rows);
@@ -1311,6 +1426,8 @@ void cgp_graph_destroy(cgp_graph* g) {
   (void)hipFree(g->off);
   (void)hipFree(g->idx);
   (void)hipFree(g->rows);
+  (void)hipFree(g->pull_lo);
+  (void)hipFree(g->pull_hi);
   if (g->wt) (void)hipFree(g->wt);
   delete g;
 }
@@ -1381,6 +1498,7 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     HIP_CHECK(hipHostMalloc((void**)&en->h_ctl, sizeof(Ctl), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void**)&en->h_cfg, sizeof(Cfg), hipHostMallocDefault));
     HIP_CHECK(hipMemset(en->d_ctl, 0, sizeof(Ctl)));
+    en->iters.ensure(sizeof(IterRec) * (size_t)ITER_RING);
     *out = en.release();
   });
 }
@@ -1392,7 +1510,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   DevBuf* bufs[] = {&en->visited, &en->next, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
                     &en->large[1], &en->dense[0], &en->dense[1], &en->parent, &en->wcc_rows,
-                    &en->outbuf, &en->qmap, &en->slot_src};
+                    &en->outbuf, &en->qmap, &en->slot_src, &en->iters};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
   if (en->d_ctl) (void)hipFree(en->d_ctl);
@@ -1418,6 +1536,30 @@ cgp_status cgp_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, ui
     std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
     if (!lk.owns_lock() || en->serve) cgpi::fail(CGP_EBUSY, "cgp_run_batch: engine is busy");
     cgpi::engine_run_batch(en, queries, n, out, out_on_device != 0, stats);
+  });
+}
+
+cgp_status cgp_last_run_report(const cgp_engine* en, cgp_iteration_record* out, int32_t cap, int32_t* n) {
+  CGP_TRY({
+    if (!en || !n || (cap > 0 && !out)) cgpi::fail(CGP_EINVAL, "cgp_last_run_report: null argument");
+    const double khz = en->clock_khz;
+    const int32_t m = (int32_t)std::min<size_t>(en->report.size(), (size_t)std::max(cap, 0));
+    for (int32_t i = 0; i < m; ++i) {
+      const IterRec& r = en->report[(size_t)i];
+      cgp_iteration_record& o = out[i];
+      o.pass = (int32_t)(r.pull >> 16);
+      o.iteration = (int32_t)r.iter;
+      o.pull = (int32_t)(r.pull & 0xFFFF);
+      o.dense = (int32_t)r.dense;
+      o.frontier_items = r.frontier_items;
+      o.touched = r.touched;
+      o.next_items = r.next_items;
+      o.next_edges = (int64_t)r.next_edges;
+      o.next_bfs_pairs = (int64_t)r.next_bfs_pairs;
+      o.advance_ms = (double)r.adv_ticks / khz;
+      o.filter_ms = (double)r.flt_ticks / khz;
+    }
+    *n = cap > 0 ? m : (int32_t)en->report.size();
   });
 }
 

@@ -41,6 +41,11 @@ struct cgp_graph {
   int32_t* idx = nullptr;
   int32_t* rows = nullptr;
   u32* wt = nullptr;
+  // Static degree split of the vertices with out-edges, for bottom-up steps.
+  int32_t* pull_lo = nullptr;
+  int32_t* pull_hi = nullptr;
+  int32_t n_pull_lo = 0;
+  int32_t n_pull_hi = 0;
 };
 
 namespace cgpi {
@@ -97,6 +102,8 @@ struct cgp_engine {
   int cus = 1;
   int wcap = 0;            // words allocated in visited/next/front/dense
   cgpi::DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
+  cgpi::DevBuf iters;  // per-iteration record ring (ITER_RING entries)
+  std::vector<cgpk::IterRec> report;  // records of the last batch run, all passes
   cgpi::DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
   Ctl* d_ctl = nullptr;
   Cfg* d_cfg = nullptr;
@@ -134,6 +141,10 @@ struct cgp_engine {
     P.idx = g->idx;
     P.wt = g->weighted ? g->wt : nullptr;
     P.rows = g->rows;
+    P.pull_lo = g->pull_lo;
+    P.pull_hi = g->pull_hi;
+    P.n_pull_lo = g->n_pull_lo;
+    P.n_pull_hi = g->n_pull_hi;
     P.visited = (u64*)visited.p;
     P.next = (u64*)next.p;
     P.front = (u64*)front.p;
@@ -148,6 +159,7 @@ struct cgp_engine {
     }
     P.ctl = d_ctl;
     P.cfg = d_cfg;
+    P.iters = (cgpk::IterRec*)iters.p;
     P.bin_small = opts.bin_small;
     P.bin_large = opts.bin_large;
     return P;

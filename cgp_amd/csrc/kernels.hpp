@@ -42,6 +42,7 @@ constexpr int BLOCK = 256;       // workgroup size of every kernel (4 waves)
 constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
 constexpr int DCHUNK = 512;      // edges per dense lane-per-query wave item
 constexpr int NPROF = 12;
+constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
 
 // Per-batch slot configuration (device resident, rewritten per pass).
 struct Cfg {
@@ -73,7 +74,7 @@ struct Ctl {
   u32 dense_iters;
   u32 pad;
   u32 bcount[2][4];           // [parity][small, medium, large items, dense items]
-  u32 fetch[2][4];            // [parity] dynamic work cursors of the advance (L, D, M, S)
+  u32 fetch[2][5];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high)
   u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
   u64 bfs_edges[2];           // ... weighted by degree
   u64 fedges[2];              // summed degree of the next frontier's vertices
@@ -87,6 +88,22 @@ struct Ctl {
   u64 adv_ticks, flt_ticks, adv_launches;
 };
 
+// Per-iteration record, written by the filter's last workgroup into a ring
+// of ITER_RING entries (index = iteration % ITER_RING), no host sync.
+constexpr int ITER_RING = 65536;
+struct IterRec {
+  u32 iter;
+  u32 pull;            // BFS bits went bottom-up this iteration
+  u32 dense;           // dense frontier mode (fire-and-forget atomics, full filter scan)
+  u32 frontier_items;  // work items of the frontier the advance consumed
+  u32 touched;         // vertices the filter visited (V in dense mode)
+  u32 next_items;      // work items of the frontier the filter produced
+  u64 next_edges;      // summed degree of that frontier
+  u64 next_bfs_pairs;  // its BFS (vertex, query) pairs
+  u64 adv_ticks;       // advance duration, device wall-clock ticks
+  u64 flt_ticks;       // filter duration
+};
+
 // Static per-engine kernel parameters (buffer addresses and tuning).
 struct KP {
   int32_t V;
@@ -96,6 +113,10 @@ struct KP {
   const int32_t* idx;
   const u32* wt;
   const int32_t* rows;
+  const int32_t* pull_lo;   // vertices with 1 <= degree <= PULL_WAVE_DEG (lane per vertex)
+  const int32_t* pull_hi;   // vertices with degree > PULL_WAVE_DEG (wave per vertex)
+  int32_t n_pull_lo;
+  int32_t n_pull_hi;
   u64* visited;
   u64* next;
   u64* front;
@@ -108,6 +129,7 @@ struct KP {
   u64* dense[2];
   Ctl* ctl;
   const Cfg* cfg;
+  IterRec* iters;
   int32_t bin_small;
   int32_t bin_large;
 };

@@ -185,6 +185,28 @@ cgp_status cgp_engine_get_opts(const cgp_engine* e, cgp_engine_opts* out);
 cgp_status cgp_run_batch(cgp_engine* e, const cgp_query* queries, int32_t n, uint32_t* out,
                          int32_t out_on_device, cgp_run_stats* stats);
 
+/* One record per frontier iteration of the last cgp_run_batch, written on the
+ * device by the compaction kernel (no host sync); replaces iteration_profile_t /
+ * level_profile_t (puercgp/include/puercgp/core/result.hxx:21-46,
+ * cgp/include/gunrock/algorithms/cgp_bfs.hxx:107-129). */
+typedef struct {
+  int32_t pass;              /* word-group pass of the batch */
+  int32_t iteration;
+  int32_t pull;              /* 1: BFS bits went bottom-up */
+  int32_t dense;             /* 1: dense frontier mode (full compaction scan) */
+  int64_t frontier_items;    /* work items the advance consumed */
+  int64_t touched;           /* vertices the compaction visited */
+  int64_t next_items;        /* work items of the next frontier */
+  int64_t next_edges;        /* summed degree of the next frontier */
+  int64_t next_bfs_pairs;    /* BFS (vertex, query) pairs of the next frontier */
+  double advance_ms;         /* device-clock duration of the advance launch */
+  double filter_ms;          /* ... of the compaction launch */
+} cgp_iteration_record;
+
+/* Copies up to cap records of the last batch run; *n = records copied
+ * (cap == 0: *n = records available). */
+cgp_status cgp_last_run_report(const cgp_engine* e, cgp_iteration_record* out, int32_t cap, int32_t* n);
+
 /* Per-query edge counts m_q = sum of out-degrees of the vertices reached by query q
  * (out[q*V+v] != CGP_INF); device-resident results in, n counts out (host). */
 cgp_status cgp_reached_edges(const cgp_graph* g, const uint32_t* out_device, int32_t n,

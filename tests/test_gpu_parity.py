@@ -230,3 +230,47 @@ def test_properties_at_scale_rmat20_sssp(gpu):
         assert check_sssp(og.off, og.idx, og.wt, int(src[i]), out[i]) is None
     for i in (0, 63):
         assert np.array_equal(out[i], O.sssp(og, src[i]))
+
+
+def test_grid_512_sssp_eight_words(gpu):
+    # The high-diameter workload's shape at a small size: a 128 x 128 grid with
+    # one random extra edge per vertex w.p. 0.01, weights in [1,1000], 512 SSSP
+    # queries = 8 mask words in one pass (long tails of small frontiers).
+    h = cgp_amd.HostCSR.grid(128, 128, p_extra=0.01, seed=5, wmin=1, wmax=1000)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = h.sample_sources(512, seed=5)
+    out, st = cgp_amd.Engine(g).run_batch([SSSP] * 512, src)
+    assert st["passes"] == 1 and st["iterations"] > 50
+    for i in range(512):
+        assert check_sssp(og.off, og.idx, og.wt, int(src[i]), out[i]) is None, i
+    for i in (0, 137, 511):
+        assert np.array_equal(out[i], O.sssp(og, src[i]))
+
+
+def test_grid_bfs_long_tail_sparse_mode(gpu):
+    # A plain path-like grid (no shortcuts): BFS depths reach 254, hundreds of
+    # iterations with small frontiers in sparse mode.
+    h = cgp_amd.HostCSR.grid(128, 128, p_extra=0.0, seed=1)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = [0, 128 * 128 - 1, 64 * 128 + 64]
+    out, st = cgp_amd.Engine(g).run_batch([BFS, BFS, SSSP], src)
+    assert st["iterations"] >= 254
+    assert np.array_equal(out[0], O.bfs(og, 0))
+    assert np.array_equal(out[1], O.bfs(og, src[1]))
+    assert np.array_equal(out[2], O.sssp(og, src[2], unit=True))
+
+
+def test_run_report_per_iteration(gpu):
+    h, g, og = rmat(12, 16, 3)
+    src = h.sample_sources(40, seed=40)
+    en = cgp_amd.Engine(g, profile=1)
+    out, st = en.run_batch([BFS] * 20 + [SSSP] * 20, src)
+    rep = en.last_run_report()
+    assert len(rep) == st["iterations"]
+    assert [r["iteration"] for r in rep] == list(range(st["iterations"]))
+    assert rep[0]["frontier_items"] >= 1 and rep[-1]["next_items"] == 0
+    assert all(rep[i]["next_items"] == rep[i + 1]["frontier_items"] for i in range(len(rep) - 1))
+    assert abs(sum(r["advance_ms"] for r in rep) - st["advance_ms"]) < 1e-6 + 1e-6 * st["advance_ms"]
+    assert any(r["dense"] for r in rep) and not rep[0]["dense"]
