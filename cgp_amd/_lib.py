@@ -64,6 +64,9 @@ class cgp_run_stats(C.Structure):
         ("advance_bytes", C.c_double),
         ("frontier_vertices", C.c_int64),
         ("edges_scanned", C.c_int64),
+        ("dense_ms", C.c_double),
+        ("dense_launches", C.c_int64),
+        ("dense_bytes", C.c_double),
     ]
 
     def as_dict(self):

@@ -94,6 +94,17 @@ __device__ __forceinline__ bool claim_vertex(u32* mark, int32_t v, u32 stamp) {
   return atomicExch(&mark[v], stamp) != stamp;
 }
 
+// A wave-uniform pointer into global memory pinned to scalar registers, so
+// accesses through it use the SGPR-base + 32-bit lane-offset addressing form
+// (one VGPR per access instead of a 64-bit per-lane address).
+typedef __attribute__((address_space(1))) u32 gu32;
+__device__ __forceinline__ gu32* uniform_gptr(const u32* p) {
+  const u64 a = (u64)p;
+  const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)a);
+  const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(a >> 32));
+  return (gu32*)(((u64)hi << 32) | lo);
+}
+
 __device__ __forceinline__ size_t vrow(int32_t V, int w, int32_t v, int rshift) {
   return ((size_t)w * (size_t)V + (size_t)v) << rshift;
 }
@@ -344,6 +355,10 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   const u32 total = (u32)__shfl((int)incl, 63, 64);
   const u32 smlo = (u32)sm, smhi = (u32)(sm >> 32);
   const bool inrow = lane < R;
+  // Lane offset inside a value row; every lane reads a valid address of the
+  // (always valid) row, inactive lanes just discard what they read, so row
+  // indices stay wave-uniform.
+  const int lq = lane & (R - 1);
   for (u32 win = 0; win < total; win += 64) {
     const u32 f = win + (u32)lane;
     const int j = wave_upper_bound(incl, f);
@@ -370,11 +385,13 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
         const u64 smk = ((u64)(u32)__builtin_amdgcn_readlane((int)smhi, it) << 32) |
                         (u32)__builtin_amdgcn_readlane((int)smlo, it);
         act[k] = (k0 + k < cnt) && inrow && ((smk >> lane) & 1ull);
-        va[k] = act[k] ? (u32)wk * (u32)P.V + (u32)vk : 0u;
-        const u32 ur = act[k] ? (u32)wk * (u32)P.V + (u32)uk : 0u;
+        va[k] = (u32)wk * (u32)P.V + (u32)vk;
+        const u32 ur = (u32)wk * (u32)P.V + (u32)uk;
         cand[k] = (u32)__builtin_amdgcn_readlane((int)myw, jj);
-        du[k] = P.vals[((size_t)ur << rshift) + (act[k] ? lane : 0)];
-        cur[k] = P.vals[((size_t)va[k] << rshift) + (act[k] ? lane : 0)];
+        const gu32* urow = uniform_gptr(P.vals + ((size_t)ur << rshift));
+        const gu32* vrw = uniform_gptr(P.vals + ((size_t)va[k] << rshift));
+        du[k] = urow[(unsigned)lq];
+        cur[k] = vrw[(unsigned)lq];
       }
     };
     issue(0);
@@ -397,7 +414,9 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
       u64 mine = 0;
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        if (imp[k]) atomicMin(&P.vals[((size_t)va_c[k] << rshift) + lane], cand_c[k]);
+        if (imp[k])
+          __hip_atomic_fetch_min(uniform_gptr(P.vals + ((size_t)va_c[k] << rshift)) + (unsigned)lq, cand_c[k],
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const u64 m = __ballot(imp[k]);
         if (!SPARSE) {
           mine = (lane == ((k0 + k) & 63)) ? m : mine;
@@ -540,7 +559,7 @@ __device__ __forceinline__ u32 wave_fetch(u32* ctr, u32 n) {
 //   S  64 low-degree vertices per wave, edges flattened, lane = edge
 //   P  (pull iterations) one lane per vertex of the whole graph
 template <int W, bool SPARSE, int BS>
-__device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t) {
+__device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tally& t) {
   Ctl* C = P.ctl;
   const u32 iter = C->iter, p = iter & 1u;
   const bool bfs_on = C->bfs_pull == 0;
@@ -561,13 +580,6 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
     const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
     expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, dense_t, rshift,
                             stamp, t);
-  }
-  // Phase D.
-  const u32 nD = C->bcount[p][3];
-  while (nD) {
-    const u32 i = wave_fetch(&C->fetch[p][1], 64u);
-    if (i >= nD) break;
-    expand_flat_dense<W, SPARSE>(P, i, nD, rshift, stamp, t);
   }
   // Phase M.
   const u32 nM = C->bcount[p][1];
@@ -610,31 +622,73 @@ __device__ __forceinline__ void advance_body(const KP& P, u32* s_item, Tally& t)
   }
 }
 
-// The advance of one iteration, as one persistent launch of BS-thread
-// workgroups (BS is a tuning parameter: 128, 256 or 512).
+// Flushes a lane's work counters (profile mode).
+__device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
+  prof_add(P, PF_PUSH_ITEMS, t.items);
+  prof_add(P, PF_PUSH_EDGES, t.edges);
+  prof_add(P, PF_PUSH_WEDGES, t.wedges);
+  prof_add(P, PF_BFS_WORDS, t.bfs);
+  prof_add(P, PF_SSSP_PAIRS, t.sssp);
+  prof_add(P, PF_NEW_WORDS, t.nnew);
+  prof_add(P, PF_IMPROVED, t.imp);
+  prof_add(P, PF_DENSE_ITEMS, t.ditems);
+  prof_add(P, PF_DENSE_EDGES, t.dedges);
+  prof_add(P, PF_PULL_VERTS, t.pverts);
+  prof_add(P, PF_PULL_WORDS, t.pwords);
+  prof_add(P, PF_PULL_EDGES, t.pedges);
+}
+
+// The advance of one iteration is two persistent launches of BS-thread
+// workgroups (BS is a tuning parameter: 128, 256 or 512).  k_adv_dense runs
+// the lane-per-query rows of dense SSSP words (phase D) with its own lean
+// register budget, one instantiation per frontier mode (the one that does
+// not match the device-selected mode exits at once); k_adv_push runs phases
+// L, M, S and P.
+template <int W, int BS, bool SPARSE>
+__global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished || (C->dense_mode == 0) != SPARSE) return;
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 nD = C->bcount[p][3];
+  if (nD == 0) return;
+  if (threadIdx.x == 0) {
+    const u64 now = (u64)wall_clock64();
+    atomicMin(&C->adv_t0, now);
+    atomicMin(&C->den_t0, now);
+  }
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift;
+  Tally t;
+  while (true) {
+    const u32 i = wave_fetch(&C->fetch[p][1], 64u);
+    if (i >= nD) break;
+    expand_flat_dense<W, SPARSE>(P, i, nD, rshift, stamp, t);
+  }
+  if (P.profile) {
+    prof_add(P, PF_DENSE_ITEMS, t.ditems);
+    prof_add(P, PF_DENSE_EDGES, t.dedges);
+    prof_add(P, PF_DENSE_PAIRS, t.sssp);
+    prof_add(P, PF_DENSE_IMP, t.imp);
+    prof_add(P, PF_DENSE_NEW, t.nnew);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u64 now = (u64)wall_clock64();
+    atomicMax(&C->adv_t1, now);
+    atomicMax(&C->den_t1, now);
+  }
+}
+
 template <int W, int BS>
-__global__ __launch_bounds__(BS) void k_advance(KP P) {
+__global__ __launch_bounds__(BS) void k_adv_push(KP P) {
   Ctl* C = P.ctl;
   if (C->finished) return;
   __shared__ u32 s_item;
   if (threadIdx.x == 0) atomicMin(&C->adv_t0, (u64)wall_clock64());
   Tally t;
-  if (C->dense_mode) advance_body<W, false, BS>(P, &s_item, t);
-  else advance_body<W, true, BS>(P, &s_item, t);
-  if (P.profile) {
-    prof_add(P, PF_PUSH_ITEMS, t.items);
-    prof_add(P, PF_PUSH_EDGES, t.edges);
-    prof_add(P, PF_PUSH_WEDGES, t.wedges);
-    prof_add(P, PF_BFS_WORDS, t.bfs);
-    prof_add(P, PF_SSSP_PAIRS, t.sssp);
-    prof_add(P, PF_NEW_WORDS, t.nnew);
-    prof_add(P, PF_IMPROVED, t.imp);
-    prof_add(P, PF_DENSE_ITEMS, t.ditems);
-    prof_add(P, PF_DENSE_EDGES, t.dedges);
-    prof_add(P, PF_PULL_VERTS, t.pverts);
-    prof_add(P, PF_PULL_WORDS, t.pwords);
-    prof_add(P, PF_PULL_EDGES, t.pedges);
-  }
+  if (C->dense_mode) advance_push_body<W, false, BS>(P, &s_item, t);
+  else advance_push_body<W, true, BS>(P, &s_item, t);
+  if (P.profile) flush_tally(P, t);
   __syncthreads();
   if (threadIdx.x == 0) atomicMax(&C->adv_t1, (u64)wall_clock64());
 }
@@ -822,9 +876,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     // ran iff iter != seeding.
     if (iter != 0xFFFFFFFFu) {
       const u64 at = C->adv_t1 - C->adv_t0, ft = (u64)wall_clock64() - C->flt_t0;
+      const u64 dt = C->den_t1 > C->den_t0 ? C->den_t1 - C->den_t0 : 0ull;
       C->adv_ticks += at;
       C->adv_launches += 1;
       C->flt_ticks += ft;
+      if (dt) {
+        C->den_ticks += dt;
+        C->den_launches += 1;
+      }
       if (P.iters) {
         IterRec& r = P.iters[iter % ITER_RING];
         r.iter = iter;
@@ -837,10 +896,13 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
         r.next_bfs_pairs = fp;
         r.adv_ticks = at;
         r.flt_ticks = ft;
+        r.den_ticks = dt;
       }
     }
     C->adv_t0 = ~0ull;
     C->adv_t1 = 0;
+    C->den_t0 = ~0ull;
+    C->den_t1 = 0;
     C->flt_t0 = ~0ull;
     const u32 next_iter = iter + 1u;
     C->iter = next_iter;
@@ -983,9 +1045,19 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
   const int bs = en->opts.block_size;
   const dim3 grid(en->grid_advance());
-  if (bs == 128) hipLaunchKernelGGL((k_advance<W, 128>), grid, dim3(128), 0, s, P);
-  else if (bs == 512) hipLaunchKernelGGL((k_advance<W, 512>), grid, dim3(512), 0, s, P);
-  else hipLaunchKernelGGL((k_advance<W, 256>), grid, dim3(256), 0, s, P);
+  if (bs == 128) {
+    hipLaunchKernelGGL((k_adv_dense<W, 128, false>), grid, dim3(128), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, 128, true>), grid, dim3(128), 0, s, P);
+    hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
+  } else if (bs == 512) {
+    hipLaunchKernelGGL((k_adv_dense<W, 512, false>), grid, dim3(512), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, 512, true>), grid, dim3(512), 0, s, P);
+    hipLaunchKernelGGL((k_adv_push<W, 512>), grid, dim3(512), 0, s, P);
+  } else {
+    hipLaunchKernelGGL((k_adv_dense<W, 256, false>), grid, dim3(256), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, 256, true>), grid, dim3(256), 0, s, P);
+    hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, s, P);
+  }
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
   HIP_CHECK(hipGetLastError());
 }
@@ -1118,7 +1190,12 @@ void add_work_bytes(cgp_run_stats& st, const u64* pf, int W) {
                       (double)pf[PF_SSSP_PAIRS] * 4.0 + (double)pf[PF_IMPROVED] * 4.0 +
                       (double)pf[PF_NEW_WORDS] * 16.0 + (double)pf[PF_DENSE_ITEMS] * 16.0 +
                       (double)pf[PF_DENSE_EDGES] * 8.0 + (double)pf[PF_PULL_VERTS] * (8.0 + Wd) +
-                      (double)pf[PF_PULL_EDGES] * 4.0 + (double)pf[PF_PULL_WORDS] * 8.0;
+                      (double)pf[PF_PULL_EDGES] * 4.0 + (double)pf[PF_PULL_WORDS] * 8.0 +
+                      (double)pf[PF_DENSE_PAIRS] * 4.0 + (double)pf[PF_DENSE_IMP] * 4.0 +
+                      (double)pf[PF_DENSE_NEW] * 16.0;
+  st.dense_bytes += (double)pf[PF_DENSE_ITEMS] * 16.0 + (double)pf[PF_DENSE_EDGES] * 8.0 +
+                    (double)pf[PF_DENSE_PAIRS] * 4.0 + (double)pf[PF_DENSE_IMP] * 4.0 +
+                    (double)pf[PF_DENSE_NEW] * 16.0;
   st.frontier_vertices += (int64_t)(pf[PF_PUSH_ITEMS] + pf[PF_DENSE_ITEMS]);
   st.edges_scanned += (int64_t)(pf[PF_PUSH_EDGES] + pf[PF_DENSE_EDGES] + pf[PF_PULL_EDGES]);
 }
@@ -1218,6 +1295,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     std::memset(&c, 0, sizeof(Ctl));
     c.iter = 0xFFFFFFFFu;
     c.adv_t0 = ~0ull;
+    c.den_t0 = ~0ull;
     c.flt_t0 = ~0ull;
     if (en->stamp_epoch > 0xF0000000u) {
       HIP_CHECK(hipMemsetAsync(en->mark.p, 0, (size_t)V * 4, en->s));
@@ -1266,6 +1344,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       st.advance_ms += (double)hc.adv_ticks / en->clock_khz;
       st.filter_ms += (double)hc.flt_ticks / en->clock_khz;
       st.advance_launches += (int64_t)hc.adv_launches;
+      st.dense_ms += (double)hc.den_ticks / en->clock_khz;
+      st.dense_launches += (int64_t)hc.den_launches;
     }
     en->stamp_epoch += hc.iter + 2;
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run

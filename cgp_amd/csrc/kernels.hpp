@@ -41,7 +41,7 @@ constexpr u32 INF = 0xFFFFFFFFu;
 constexpr int BLOCK = 256;       // workgroup size of every kernel (4 waves)
 constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
 constexpr int DCHUNK = 512;      // edges per dense lane-per-query wave item
-constexpr int NPROF = 12;
+constexpr int NPROF = 15;
 constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
 
 // Per-batch slot configuration (device resident, rewritten per pass).
@@ -84,8 +84,8 @@ struct Ctl {
   // Device-clock launch timing (wall_clock64 ticks): first workgroup start
   // and last workgroup end of the running advance / filter, and the sums
   // over the launches of the run that did work.
-  u64 adv_t0, adv_t1, flt_t0;
-  u64 adv_ticks, flt_ticks, adv_launches;
+  u64 adv_t0, adv_t1, flt_t0, den_t0, den_t1;
+  u64 adv_ticks, flt_ticks, adv_launches, den_ticks, den_launches;
 };
 
 // Per-iteration record, written by the filter's last workgroup into a ring
@@ -100,8 +100,9 @@ struct IterRec {
   u32 next_items;      // work items of the frontier the filter produced
   u64 next_edges;      // summed degree of that frontier
   u64 next_bfs_pairs;  // its BFS (vertex, query) pairs
-  u64 adv_ticks;       // advance duration, device wall-clock ticks
+  u64 adv_ticks;       // advance duration (both launches), device wall-clock ticks
   u64 flt_ticks;       // filter duration
+  u64 den_ticks;       // dense-SSSP advance launch duration (0 when it had no work)
 };
 
 // Static per-engine kernel parameters (buffer addresses and tuning).
@@ -147,7 +148,10 @@ enum : int {
   PF_DENSE_EDGES,      // edges scanned lane-per-query
   PF_PULL_VERTS,       // vertices examined by pull
   PF_PULL_WORDS,       // (edge, word) visited reads by pull
-  PF_PULL_EDGES        // edges scanned by pull
+  PF_PULL_EDGES,       // edges scanned by pull
+  PF_DENSE_PAIRS,      // (edge, active query) pairs of the dense launch
+  PF_DENSE_IMP,        // values lowered by the dense launch
+  PF_DENSE_NEW         // next words gaining bits in the dense launch
 };
 
 }  // namespace cgpk

@@ -106,6 +106,9 @@ typedef struct {
   double advance_bytes;        /* profile bit 1: algorithmic bytes of all advance launches */
   int64_t frontier_vertices;   /* profile bit 1: frontier items expanded */
   int64_t edges_scanned;       /* profile bit 1: edges touched by push and pull */
+  double dense_ms;             /* profile bit 0: summed time of the dense-SSSP advance launches */
+  int64_t dense_launches;      /* profile bit 0: dense-SSSP advance launches that had work */
+  double dense_bytes;          /* profile bit 1: algorithmic bytes of those launches */
 } cgp_run_stats;
 
 /* ---------------------------------------------------------------- errors */

@@ -273,4 +273,4 @@ def test_run_report_per_iteration(gpu):
     assert rep[0]["frontier_items"] >= 1 and rep[-1]["next_items"] == 0
     assert all(rep[i]["next_items"] == rep[i + 1]["frontier_items"] for i in range(len(rep) - 1))
     assert abs(sum(r["advance_ms"] for r in rep) - st["advance_ms"]) < 1e-6 + 1e-6 * st["advance_ms"]
-    assert any(r["dense"] for r in rep) and not rep[0]["dense"]
+    assert all(r["dense"] in (0, 1) and r["pull"] in (0, 1) for r in rep)
