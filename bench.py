@@ -99,6 +99,39 @@ def cpu_baseline(h, kinds, sources, seconds_target):
     }
 
 
+def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R):
+    """HBM bytes per working dispatch of `kernel_prefix` from rocprofv3
+    FETCH_SIZE / WRITE_SIZE CSVs (KiB per dispatch), each scaled by the factor
+    that makes the same counter read the known bytes of k_output (reads the
+    words*V*R value rows, writes out_rows*V*4 bytes) in the same run."""
+    import csv
+    import glob
+    tot = 0.0
+    for counter, known in (("FETCH_SIZE", 4.0 * words * V * R), ("WRITE_SIZE", 4.0 * out_rows * V)):
+        vals, cal = [], []
+        for f in glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") != counter:
+                    continue
+                name = r.get("Kernel_Name", "")
+                v = float(r.get("Counter_Value", 0.0)) * 1024.0
+                if "k_output" in name:
+                    cal.append(v)
+                elif kernel_prefix in name:
+                    vals.append(v)
+        if not vals or not cal:
+            return None, None
+        factor = known / (sum(cal) / len(cal))
+        top = max(vals)
+        work = [v for v in vals if v > 0.01 * top]
+        tot += factor * sum(work) / len(work)
+        if counter == "FETCH_SIZE":
+            fetch_factor = factor
+        else:
+            write_factor = factor
+    return tot, {"fetch_factor": fetch_factor, "write_factor": write_factor}
+
+
 def make_graph(spec, seed, wmin, wmax):
     import cgp_amd
     if spec[0] == "rmat":
@@ -117,7 +150,8 @@ def describe(spec, seed, wmin, wmax):
 def serve_bench(args):
     """Open-loop Poisson serving: 70% BFS / 30% SSSP with uniform sources;
     five arrival rates relative to the measured saturation throughput; the
-    admission cap is the HBM budget for the per-query value rows."""
+    admission cap is the HBM budget for the per-query value rows.  Latency
+    runs from a query's arrival to its result resident in host memory."""
     import numpy as np
 
     import cgp_amd
@@ -125,47 +159,63 @@ def serve_bench(args):
 
     spec, seed, wmin, wmax = SERVE[args.workload]
     t0 = time.time()
+    log("generating graph:", describe(spec, seed, wmin, wmax))
     h = make_graph(spec, seed, wmin, wmax)
+    log("graph generated in %.1fs, uploading" % (time.time() - t0))
     g = cgp_amd.Graph.from_host(h)
     V = g.V
-    deg = h.degrees()
-    log("graph: V=%d E=%d generated in %.1fs" % (h.V, h.E, time.time() - t0))
+    log("graph: V=%d E=%d ready in %.1fs" % (h.V, h.E, time.time() - t0))
     nq = args.serve_queries
     rng = np.random.default_rng(seed)
-    src = h.sample_sources(nq, seed=seed)
+    src = [int(x) for x in h.sample_sources(nq, seed=seed)]
     kinds = [BFS if x < 0.7 else SSSP for x in rng.random(nq)]
     budget = int(args.serve_budget_gib * (1 << 30))
     en = cgp_amd.Engine(g, hbm_budget_bytes=budget)
 
-    def run(arrivals, keep):
-        en.serve_start(max_slots=0, keep_results=1 if keep else 0)
+    # The same queries as one batch: m_q (edges reached, on the device) and
+    # the reference rows for the parity check of the warm-up session.
+    bout = cgp_amd.DeviceBuffer(nq * V * 4)
+    _, bst = en.run_batch(kinds, src, out=bout)
+    m_q = cgp_amd.reached_edges(g, bout, nq).astype(np.float64)
+    sum_m = float(m_q.sum())
+    batch_rows = bout.to_host(shape=(nq, V))
+    del bout
+    log("one batch of the %d queries: %.1f ms" % (nq, bst["wall_ms"]))
+
+    def run(arrivals, check=False):
+        en.serve_start(max_slots=0, keep_results=1)
         t_base = en.serve_clock()
         nxt, done = 0, {}
+        same = True
         while len(done) < nq:
             now = en.serve_clock() - t_base
             while nxt < nq and arrivals[nxt] <= now:
-                en.serve_submit(kinds[nxt], int(src[nxt]), nxt, t_base + arrivals[nxt])
+                en.serve_submit(kinds[nxt], src[nxt], nxt, t_base + arrivals[nxt])
                 nxt += 1
             if en.serve_pending() == (0, 0) and nxt < nq:
                 time.sleep(min(0.001, max(0.0, arrivals[nxt] - now)))
                 continue
-            for c in en.serve_step(max_iterations=1):
+            for c in en.serve_step(max_iterations=1, fetch=check):
+                if check:
+                    same = same and bool(np.array_equal(c["result"], batch_rows[c["ticket"]]))
+                    c["result"] = None
                 done[c["ticket"]] = c
         en.serve_stop()
-        return done
+        return done, same
 
-    # Saturation: everything arrives at t=0.
-    done = run(np.zeros(nq), True)
-    m_q = np.array([float(deg[done[i]["result"] != cgp_amd.INF].sum()) for i in range(nq)])
+    # Warm-up session (pins the host result buffers, checks parity), then
+    # saturation: everything arrives at t=0.
+    _, same = run(np.zeros(nq), check=True)
+    log("warm-up session: results equal the batch: %s" % same)
+    done, _ = run(np.zeros(nq))
     span = max(c["done_s"] for c in done.values()) - min(c["arrival_s"] for c in done.values())
     sat_qps = nq / span
-    sum_m = float(m_q.sum())
     log("saturation: %.1f queries/s, %.2f query-GTEPS" % (sat_qps, sum_m / span / 1e9))
     rates = []
     for frac in (0.2, 0.4, 0.6, 0.8, 0.95):
         lam = frac * sat_qps
         arr = cgp_amd.poisson_arrivals(nq, lam, seed=seed + int(frac * 100))
-        d = run(arr, False)
+        d, _ = run(arr)
         lat = [(d[i]["done_s"] - d[i]["arrival_s"]) * 1e3 for i in range(nq)]
         mk = max(c["done_s"] for c in d.values()) - min(c["arrival_s"] for c in d.values())
         rates.append({"rate_qps": lam, "load": frac, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
@@ -175,14 +225,17 @@ def serve_bench(args):
     best = rates[-1]
     out = {
         "metric": METRIC, "value": sum_m / span / 1e9, "unit": "query-GTEPS", "n_gpus": 1, "steps": 1,
-        "warmup": 0, "ms_per_step": span * 1e3, "higher_is_better": True, "scaling": "weak",
+        "warmup": 1, "ms_per_step": span * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": args.workload, "graph": describe(spec, seed, wmin, wmax), "V": V, "E": g.E,
                    "queries": nq, "mix": "70% BFS / 30% SSSP", "hbm_budget_bytes": budget,
-                   "slots_by_budget": min(512, budget // (4 * V))},
+                   "slots_by_budget": min(512, budget // (4 * V)),
+                   "latency": "arrival to result in host memory"},
         "saturation_qps": sat_qps,
+        "one_batch_ms": bst["wall_ms"],
         "latency_ms": {"p50_at_95pct_load": best["p50_ms"], "p99_at_95pct_load": best["p99_ms"]},
         "poisson": rates,
+        "parity": {"serving_equals_batch": same},
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -198,6 +251,8 @@ def main():
                     help="time one-at-a-time on the first N queries only and scale (0 = all)")
     ap.add_argument("--serve-queries", type=int, default=160)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--pmc-dir", default="",
+                    help="directory with rocprofv3 FETCH_SIZE/WRITE_SIZE CSVs of this workload: fills roofline.traffic")
     ap.add_argument("--no-one-at-a-time", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -216,6 +271,7 @@ def main():
         return serve_bench(args)
     spec, seed, wmin, wmax, n_bfs, n_sssp, n_wcc = WORKLOADS[args.workload]
     t0 = time.time()
+    log("generating graph:", describe(spec, seed, wmin, wmax))
     h = make_graph(spec, seed, wmin, wmax)
     log("graph: V=%d E=%d generated in %.1fs" % (h.V, h.E, time.time() - t0))
     src = h.sample_sources(n_bfs + n_sssp, seed=seed)
@@ -326,6 +382,13 @@ def main():
             "kernels": kernels_report,
         },
     }
+    if args.pmc_dir:
+        words = (n_bfs + n_sssp + 63) // 64
+        R = 64 if (n_bfs + n_sssp) > 64 else 1 << max(0, (n_bfs + n_sssp - 1).bit_length())
+        traffic, factors = pmc_traffic(args.pmc_dir, dom + "<", n_bfs + n_sssp, V, words, R)
+        if traffic:
+            result["roofline"]["traffic"] = traffic
+            result["roofline"]["traffic_calibration"] = factors
     log("batched: %.2f query-GTEPS, %.2f ms/step (sum m_q = %.3e)" % (gteps, result["ms_per_step"], sum_m))
 
     if not args.no_one_at_a_time:

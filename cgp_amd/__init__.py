@@ -303,7 +303,10 @@ class Engine:
         q = cgp_query(int(source), int(kind))
         check(lib.cgp_serve_submit(self._h, C.byref(q), int(ticket), float(arrival_s)))
 
-    def serve_step(self, max_iterations=0, cap=1024):
+    def serve_step(self, max_iterations=0, cap=1024, fetch=True):
+        """Runs up to max_iterations iterations (0: until something completes)
+        and returns the completions.  With fetch=False the host result
+        buffers are released without being copied ("result" stays None)."""
         done = (cgp_completion * cap)()
         nd = C.c_int32(0)
         check(lib.cgp_serve_step(self._h, int(max_iterations), done, cap, C.byref(nd)))
@@ -314,7 +317,8 @@ class Engine:
             rec = {"ticket": c.ticket, "kind": c.kind, "source": c.source, "arrival_s": c.arrival_s,
                    "admit_s": c.admit_s, "done_s": c.done_s, "iterations": c.iterations, "result": None}
             if c.result:
-                rec["result"] = np.ctypeslib.as_array(c.result, shape=(V,)).copy()
+                if fetch:
+                    rec["result"] = np.ctypeslib.as_array(c.result, shape=(V,)).copy()
                 lib.cgp_serve_release_result(c.result)
             out.append(rec)
         return out

@@ -45,6 +45,8 @@ class cgp_engine_opts(C.Structure):
         ("hbm_budget_bytes", C.c_uint64),
         ("dense_threshold", C.c_int32),
         ("frontier_dense_div", C.c_int32),
+        ("dense_unroll", C.c_int32),
+        ("sssp_delta", C.c_int32),
     ]
 
 

@@ -127,6 +127,22 @@ __device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
 // redundant (the vertex re-relaxes with its true value), never lose an
 // improvement, because values only decrease.
 
+// Work counters of one lane (flushed once per kernel in profile mode), and
+// the near-far statistics of its SSSP improvements (flushed whenever the
+// threshold is active).
+struct Tally {
+  u64 items = 0, edges = 0, wedges = 0, bfs = 0, sssp = 0, nnew = 0, imp = 0;
+  u64 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
+  u32 fmin = INF;  // min value of pairs postponed or improved to >= thr
+  u32 nnear = 0;   // improvements to a value < thr
+};
+
+// Near-far bookkeeping of one improvement to value c.
+__device__ __forceinline__ void note_improved(Tally& t, u32 c, u32 thr) {
+  if (c < thr) ++t.nnear;
+  else t.fmin = min(t.fmin, c);
+}
+
 // Relaxes edge (u -> v, weight wv) for every active BFS bit (bitmask
 // first-write against visited) and every sparse SSSP bit; sets the new bits
 // in next[v].  Sparse mode returns whether this lane made the vertex's first
@@ -134,8 +150,7 @@ __device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
 template <int W, bool SPARSE>
 __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u32 wv,
                                            const u64 (&bm)[W], const u64 (&sm)[W], u32 stamp,
-                                           int rshift, u64& nbfs, u64& nsssp, u64& nnew,
-                                           u64& nimp) {
+                                           int rshift, u32 thr, Tally& t) {
   bool first = false;
   const size_t vb = (size_t)v * W;
   // All W visited / next words of v in one go (each array's W words are one
@@ -151,7 +166,7 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
     if (!(bm[w] | sm[w])) continue;
     u64 add = 0;
     if (bm[w]) {
-      ++nbfs;
+      ++t.bfs;
       add = bm[w] & ~vis[w];
     }
     u64 s = sm[w];
@@ -173,19 +188,20 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           if (!a[i]) continue;
-          ++nsssp;
+          ++t.sssp;
           const u32 cand = du[i] + wv;
           if (cand < du[i]) continue;  // saturate: never wrap past CGP_INF
           if (cand < cur[i]) {
             atomicMin(&P.vals[vr + q[i]], cand);
             add |= 1ull << q[i];
-            ++nimp;
+            ++t.imp;
+            note_improved(t, cand, thr);
           }
         }
       }
     }
     if (add & ~nx[w]) {
-      ++nnew;
+      ++t.nnew;
       if (SPARSE) {
         if (atomicOr(&P.next[vb + w], add) == 0) first = true;
       } else {
@@ -196,9 +212,13 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
   return SPARSE && first && claim_vertex(P.mark, v, stamp);
 }
 
+// Frontier masks of vertex u: BFS bits, and the sparse SSSP bits (popcount
+// below dense_t; denser words go lane-per-query) split at the near-far
+// threshold: pairs whose value is >= thr land in far[] and are not expanded.
 template <int W>
 __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, int dense_t,
-                                           u64 (&bm)[W], u64 (&sm)[W], bool& any_w) {
+                                           u32 thr, int rshift, u64 (&bm)[W], u64 (&sm)[W],
+                                           u64 (&far)[W], bool& any_w, Tally& t) {
   const Cfg* cfg = P.cfg;
   any_w = false;
 #pragma unroll
@@ -207,26 +227,68 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, 
     bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
     u64 s = f & cfg->val_bits[w];
     if (__popcll(s) >= dense_t) s = 0;  // handled lane-per-query
+    far[w] = 0;
+    if (s && thr != INF) {
+      const size_t ub = vrow(P.V, w, u, rshift);
+      u64 x = s;
+      while (x) {
+        int q[4];
+        bool a[4];
+        u32 d[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a[i] = x != 0;
+          q[i] = a[i] ? __ffsll((long long)x) - 1 : 0;
+          if (a[i]) x &= x - 1;
+          d[i] = P.vals[ub + q[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (a[i] && d[i] >= thr) {
+            far[w] |= 1ull << q[i];
+            t.fmin = min(t.fmin, d[i]);
+          }
+        }
+      }
+      s &= ~far[w];
+    }
     sm[w] = s;
     any_w |= (s != 0);
   }
 }
 
-// Work counters of one lane, flushed once per kernel in profile mode.
-struct Tally {
-  u64 items = 0, edges = 0, wedges = 0, bfs = 0, sssp = 0, nnew = 0, imp = 0;
-  u64 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
-};
+// Keeps u's postponed SSSP pairs in the next frontier.  Sparse mode returns
+// whether this made the vertex's first claim of the iteration.
+template <int W, bool SPARSE>
+__device__ __forceinline__ bool postpone(const KP& P, int32_t u, const u64 (&far)[W], u32 stamp,
+                                         Tally& t) {
+  bool first = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (!far[w]) continue;
+    ++t.nnew;
+    if (SPARSE) {
+      if (atomicOr(&P.next[(size_t)u * W + w], far[w]) == 0) first = true;
+    } else {
+      atomicOr(&P.next[(size_t)u * W + w], far[w]);
+    }
+  }
+  return SPARSE && first && claim_vertex(P.mark, u, stamp);
+}
 
 // Expands frontier vertex u over edges [e0, e1) with lanes striding by
 // `stride` from `first` (lane-per-edge): BFS bits and sparse SSSP words.
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0, int32_t e1,
                                              int32_t first, int32_t stride, bool bfs_on, int dense_t,
-                                             int rshift, u32 stamp, Tally& t) {
-  u64 bm[W], sm[W];
+                                             int rshift, u32 stamp, u32 thr, bool lead, Tally& t) {
+  u64 bm[W], sm[W], far[W];
   bool any_s;
-  load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+  load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
+  if (lead) {
+    const bool c = postpone<W, SPARSE>(P, u, far, stamp, t);
+    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount, P.touched);
+  }
   bool any_b = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
@@ -240,8 +302,7 @@ __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0,
       ++t.wedges;
     }
     ++t.edges;
-    const bool claimed =
-        relax_edge<W, SPARSE>(P, u, v, wv, bm, sm, stamp, rshift, t.bfs, t.sssp, t.nnew, t.imp);
+    const bool claimed = relax_edge<W, SPARSE>(P, u, v, wv, bm, sm, stamp, rshift, thr, t);
     if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
   }
 }
@@ -277,17 +338,21 @@ __device__ __forceinline__ int wave_upper_bound(u32 incl, u32 f) {
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 base, u32 n,
                                                      bool bfs_on, int dense_t, int rshift, u32 stamp,
-                                                     Tally& t) {
+                                                     u32 thr, Tally& t) {
   const int lane = lane_id();
   const bool have = base + (u32)lane < n;
   const int32_t u = have ? list[base + lane] : 0;
-  u64 bm[W], sm[W];
+  u64 bm[W], sm[W], far[W];
   bool any_s = false, any_b = false;
-  if (have) load_masks<W>(P, u, bfs_on, dense_t, bm, sm, any_s);
+  if (have) load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    if (!have) bm[w] = sm[w] = 0;
+    if (!have) bm[w] = sm[w] = far[w] = 0;
     any_b |= bm[w] != 0;
+  }
+  {
+    const bool c = have && postpone<W, SPARSE>(P, u, far, stamp, t);
+    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount, P.touched);
   }
   const bool work = have && (any_b || any_s);
   const int32_t e0 = work ? P.off[u] : 0;
@@ -319,7 +384,7 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
         ++t.wedges;
       }
       ++t.edges;
-      claimed = relax_edge<W, SPARSE>(P, uj, v, wv, bj, sj, stamp, rshift, t.bfs, t.sssp, t.nnew, t.imp);
+      claimed = relax_edge<W, SPARSE>(P, uj, v, wv, bj, sj, stamp, rshift, thr, t);
     }
     if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
   }
@@ -331,10 +396,9 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
 // address and discard it) so the U source rows and U target rows of a step
 // are all in flight together, and the loads of step k+1 are issued before
 // the atomics of step k, so waiting for them never waits on the atomics.
-template <int W, bool SPARSE>
+template <int W, bool SPARSE, int U>
 __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
-                                                  Tally& t) {
-  constexpr int U = 8;
+                                                  u32 thr, Tally& t) {
   const int lane = lane_id();
   const int R = 1 << rshift;
   const u32 p = P.ctl->iter & 1u;
@@ -342,10 +406,60 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   const u64 item = have ? P.dense[p][base + lane] : 0ull;
   const int32_t u = (int32_t)(item >> 32);
   const int w = (int)((item >> 24) & 0xFF);
-  const int32_t e0 = have ? P.off[u] + (int32_t)((u32)(item & 0xFFFFFF) * DCHUNK) : 0;
+  const u32 chunk = (u32)(item & 0xFFFFFF);
+  const int32_t e0 = have ? P.off[u] + (int32_t)(chunk * DCHUNK) : 0;
   const int32_t e1 = have ? min(P.off[u + 1], e0 + DCHUNK) : 0;
-  const u64 sm = have ? (P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) : 0ull;
-  const u32 d = (u32)(e1 - e0);
+  u64 sm = have ? (P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) : 0ull;
+  const bool inrow = lane < R;
+  // Lane offset inside a value row; every lane reads a valid address of the
+  // (always valid) row, inactive lanes just discard what they read, so row
+  // indices stay wave-uniform.
+  const int lq = lane & (R - 1);
+  if (thr != INF) {
+    // Near-far split of the wave's items: lane q reads value q of each
+    // item's source row (8 rows in flight); pairs at or above thr are
+    // postponed (chunk 0 of the vertex keeps them in the next frontier) and
+    // an item left with no near pair scans no edges.
+    const int nit = (int)min(64u, n - base);
+    u64 near = sm;
+    for (int i0 = 0; i0 < nit; i0 += 8) {
+      u32 dv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int it = min(i0 + k, nit - 1);
+        const u32 uk = (u32)__builtin_amdgcn_readlane(u, it);
+        const u32 wk = (u32)__builtin_amdgcn_readlane(w, it);
+        dv[k] = uniform_gptr(P.vals + ((size_t)(wk * (u32)P.V + uk) << rshift))[(unsigned)lq];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int it = i0 + k;
+        if (it >= nit) break;
+        const u64 smk = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(sm >> 32), it) << 32) |
+                        (u32)__builtin_amdgcn_readlane((int)(u32)sm, it);
+        const bool isfar = inrow && ((smk >> lane) & 1ull) && dv[k] >= thr;
+        const u64 fm = __ballot(isfar);
+        if (isfar) t.fmin = min(t.fmin, dv[k]);
+        if (lane == it) near = sm & ~fm;
+        if (fm && __builtin_amdgcn_readlane((int)chunk, it) == 0) {
+          const int32_t uk = __builtin_amdgcn_readlane(u, it);
+          const int wk = __builtin_amdgcn_readlane(w, it);
+          bool claimed = false;
+          if (lane == 0) {
+            ++t.nnew;
+            if (SPARSE) {
+              if (atomicOr(&P.next[(size_t)uk * W + wk], fm) == 0) claimed = claim_vertex(P.mark, uk, stamp);
+            } else {
+              atomicOr(&P.next[(size_t)uk * W + wk], fm);
+            }
+          }
+          if (SPARSE) wave_append<int32_t>(claimed, uk, &P.ctl->tcount, P.touched);
+        }
+      }
+    }
+    sm = near;
+  }
+  const u32 d = sm ? (u32)(e1 - e0) : 0u;
   if (have) {
     ++t.ditems;
     t.dedges += d;
@@ -354,11 +468,6 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   const u32 incl = wave_incl_scan(d);
   const u32 total = (u32)__shfl((int)incl, 63, 64);
   const u32 smlo = (u32)sm, smhi = (u32)(sm >> 32);
-  const bool inrow = lane < R;
-  // Lane offset inside a value row; every lane reads a valid address of the
-  // (always valid) row, inactive lanes just discard what they read, so row
-  // indices stay wave-uniform.
-  const int lq = lane & (R - 1);
   for (u32 win = 0; win < total; win += 64) {
     const u32 f = win + (u32)lane;
     const int j = wave_upper_bound(incl, f);
@@ -414,9 +523,11 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
       u64 mine = 0;
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        if (imp[k])
+        if (imp[k]) {
           __hip_atomic_fetch_min(uniform_gptr(P.vals + ((size_t)va_c[k] << rshift)) + (unsigned)lq, cand_c[k],
                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          note_improved(t, cand_c[k], thr);
+        }
         const u64 m = __ballot(imp[k]);
         if (!SPARSE) {
           mine = (lane == ((k0 + k) & 63)) ? m : mine;
@@ -564,6 +675,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
   const u32 iter = C->iter, p = iter & 1u;
   const bool bfs_on = C->bfs_pull == 0;
   const u32 stamp = C->stamp_base + iter + 1;
+  const u32 thr = C->thr;
   const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
   const int lane = lane_id();
   // Phase L.
@@ -579,7 +691,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     const int32_t e0 = P.off[u] + (int32_t)((u32)item * LCHUNK);
     const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
     expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, dense_t, rshift,
-                            stamp, t);
+                            stamp, thr, (u32)item == 0 && threadIdx.x == 0, t);
   }
   // Phase M.
   const u32 nM = C->bcount[p][1];
@@ -588,14 +700,14 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     if (i >= nM) break;
     const int32_t u = P.medium[p][i];
     const int32_t e0 = P.off[u], e1 = P.off[u + 1];
-    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, t);
+    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
   }
   // Phase S.
   const u32 nS = C->bcount[p][0];
   while (nS) {
     const u32 base = wave_fetch(&C->fetch[p][3], 64u);
     if (base >= nS) break;
-    expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, t);
+    expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, thr, t);
   }
   // Phase P: high-degree vertices one wave each (64 neighbours per step,
   // early exit on a wave-wide OR), then the rest one lane each.
@@ -622,6 +734,20 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
   }
 }
 
+// Publishes the wave's near-far statistics: whether some improvement stayed
+// below the threshold, and the smallest value at or above it.
+__device__ __forceinline__ void flush_near_far(const KP& P, const Tally& t, u32 thr) {
+  if (thr == INF) return;
+  const u64 anyn = __ballot(t.nnear != 0);
+  u32 fm = t.fmin;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) fm = min(fm, (u32)__shfl_xor((int)fm, d, 64));
+  if (lane_id() == 0) {
+    if (anyn) atomicOr(&P.ctl->near_new, 1u);
+    if (fm != INF) atomicMin(&P.ctl->far_min, fm);
+  }
+}
+
 // Flushes a lane's work counters (profile mode).
 __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
   prof_add(P, PF_PUSH_ITEMS, t.items);
@@ -644,7 +770,7 @@ __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
 // register budget, one instantiation per frontier mode (the one that does
 // not match the device-selected mode exits at once); k_adv_push runs phases
 // L, M, S and P.
-template <int W, int BS, bool SPARSE>
+template <int W, int BS, bool SPARSE, int U>
 __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   Ctl* C = P.ctl;
   if (C->finished || (C->dense_mode == 0) != SPARSE) return;
@@ -657,13 +783,15 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
     atomicMin(&C->den_t0, now);
   }
   const u32 stamp = C->stamp_base + iter + 1;
+  const u32 thr = C->thr;
   const int rshift = P.cfg->rshift;
   Tally t;
   while (true) {
     const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
-    expand_flat_dense<W, SPARSE>(P, i, nD, rshift, stamp, t);
+    expand_flat_dense<W, SPARSE, U>(P, i, nD, rshift, stamp, thr, t);
   }
+  flush_near_far(P, t, thr);
   if (P.profile) {
     prof_add(P, PF_DENSE_ITEMS, t.ditems);
     prof_add(P, PF_DENSE_EDGES, t.dedges);
@@ -688,6 +816,7 @@ __global__ __launch_bounds__(BS) void k_adv_push(KP P) {
   Tally t;
   if (C->dense_mode) advance_push_body<W, false, BS>(P, &s_item, t);
   else advance_push_body<W, true, BS>(P, &s_item, t);
+  flush_near_far(P, t, C->thr);
   if (P.profile) flush_tally(P, t);
   __syncthreads();
   if (threadIdx.x == 0) atomicMax(&C->adv_t1, (u64)wall_clock64());
@@ -710,7 +839,10 @@ __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, i
     v = slot_src[s];
     const int w = s >> 6, q = s & 63;
     atomicOr(&P.next[(size_t)v * W + w], 1ull << q);
-    if ((P.cfg->val_bits[w] >> q) & 1ull) P.vals[vrow(P.V, w, v, P.cfg->rshift) + q] = 0u;
+    if ((P.cfg->val_bits[w] >> q) & 1ull) {
+      P.vals[vrow(P.V, w, v, P.cfg->rshift) + q] = 0u;
+      atomicOr(&C->near_new, 1u);  // a source at value 0 is below any threshold
+    }
     claimed = claim_vertex(P.mark, v, stamp);
   }
   wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
@@ -872,6 +1004,18 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       }
     }
     C->bfs_pull = pull;
+    // Near-far: when no SSSP pair below the threshold is left, move it to
+    // the smallest pending value plus delta (skipping empty buckets).
+    if (cfg->delta != 0) {
+      const u32 fm = C->far_min;
+      if (C->near_new == 0 && fm != INF) {
+        const u64 nt = (u64)fm + (u64)cfg->delta;
+        C->thr = nt >= (u64)INF ? INF : (u32)nt;
+        ++C->bumps;
+      }
+      C->near_new = 0;
+      C->far_min = INF;
+    }
     // Launch timing and the iteration record: the advance of this iteration
     // ran iff iter != seeding.
     if (iter != 0xFFFFFFFFu) {
@@ -1039,23 +1183,34 @@ __global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, int32_t V
 // ============================================================== host side
 namespace {
 
+// Both frontier-mode instantiations of the dense-SSSP advance (the one not
+// matching the device-selected mode exits at once), U = rows per step.
+template <int W, int BS>
+void launch_dense(hipStream_t s, dim3 grid, const KP& P, bool u4) {
+  if (u4) {
+    hipLaunchKernelGGL((k_adv_dense<W, BS, false, 4>), grid, dim3(BS), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BS, true, 4>), grid, dim3(BS), 0, s, P);
+  } else {
+    hipLaunchKernelGGL((k_adv_dense<W, BS, false, 8>), grid, dim3(BS), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BS, true, 8>), grid, dim3(BS), 0, s, P);
+  }
+}
+
 // One iteration: advance then filter (both time themselves on the device clock).
 template <int W>
 void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
   const int bs = en->opts.block_size;
+  const bool u4 = en->opts.dense_unroll == 4;
   const dim3 grid(en->grid_advance());
   if (bs == 128) {
-    hipLaunchKernelGGL((k_adv_dense<W, 128, false>), grid, dim3(128), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, 128, true>), grid, dim3(128), 0, s, P);
+    launch_dense<W, 128>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
   } else if (bs == 512) {
-    hipLaunchKernelGGL((k_adv_dense<W, 512, false>), grid, dim3(512), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, 512, true>), grid, dim3(512), 0, s, P);
+    launch_dense<W, 512>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 512>), grid, dim3(512), 0, s, P);
   } else {
-    hipLaunchKernelGGL((k_adv_dense<W, 256, false>), grid, dim3(256), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, 256, true>), grid, dim3(256), 0, s, P);
+    launch_dense<W, 256>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, s, P);
   }
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
@@ -1165,10 +1320,12 @@ static void run_iterations(cgp_engine* en, int W) {
     HIP_CHECK(hipStreamSynchronize(en->s));
     const u32 after = en->h_ctl->iter;
     if (en->h_ctl->finished) break;
-    // Frontier iterations are bounded by V + 1 (each one settles at least one
-    // more hop of some shortest path); more means a broken state machine.
-    if ((int64_t)after > (int64_t)en->g->V + 2)
-      cgpi::fail(CGP_EDEVICE, "traversal did not converge within V+2 iterations");
+    // Between two raises of the near-far threshold the iterations are a
+    // Bellman-Ford frontier over the pairs below it, bounded by V + 1 (each
+    // one settles at least one more hop of some shortest path); more means a
+    // broken state machine.
+    if ((int64_t)after > ((int64_t)en->h_ctl->bumps + 1) * ((int64_t)en->g->V + 2))
+      cgpi::fail(CGP_EDEVICE, "traversal did not converge within V+2 iterations per threshold step");
   }
 }
 
@@ -1256,6 +1413,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     cfg.rshift = rshift;
     cfg.W = W;
     cfg.dense_t = en->opts.dense_threshold;
+    cfg.delta = (u32)en->opts.sssp_delta;
     en->stage((size_t)W * 64 * 2);
     int32_t* h_src = en->h_stage;
     int32_t* h_qmap = en->h_stage + W * 64;
@@ -1294,6 +1452,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     Ctl& c = *en->h_ctl;
     std::memset(&c, 0, sizeof(Ctl));
     c.iter = 0xFFFFFFFFu;
+    c.thr = cfg.delta ? cfg.delta : INF;
+    c.far_min = INF;
     c.adv_t0 = ~0ull;
     c.den_t0 = ~0ull;
     c.flt_t0 = ~0ull;
@@ -1530,6 +1690,8 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->use_hip_graph = 1;
   o->dense_threshold = 16;
   o->frontier_dense_div = 8;
+  o->dense_unroll = 8;
+  o->sssp_delta = 0;
   o->graph_iterations = 8;
   o->profile = 0;
   o->hbm_budget_bytes = 64ull << 30;
@@ -1564,6 +1726,10 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
       cgpi::fail(CGP_EINVAL, "cgp_engine_create: block_size must be 128, 256 or 512");
     if (en->opts.dense_threshold <= 0) en->opts.dense_threshold = d.dense_threshold;
     if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
+    if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
+    if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8)
+      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4 or 8");
+    if (en->opts.sssp_delta < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= 0");
     en->cus = device_cus();
     {
       int dev = 0;
@@ -1586,6 +1752,7 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
 void cgp_engine_destroy(cgp_engine* en) {
   if (!en) return;
   if (en->s) (void)hipStreamSynchronize(en->s);
+  cgpi::serve_release_engine(en);
   en->drop_graphs();
   DevBuf* bufs[] = {&en->visited, &en->next, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],

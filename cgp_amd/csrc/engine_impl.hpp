@@ -131,6 +131,9 @@ struct cgp_engine {
   bool counted() const { return (opts.profile & 2) != 0; }
   // Continuous-batching session (serve.hip), null outside serving.
   ServeState* serve = nullptr;
+  // Pinned host result buffers (V words each) recycled across sessions;
+  // guarded by the serving module's pool mutex.
+  std::vector<uint32_t*> result_pool;
 
   KP kp() const {
     KP P{};
@@ -251,5 +254,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
 void compute_wcc(cgp_engine* en);
 // Writes the labels into rows rows_out[0..nrows) of out (query-major, V per row).
 void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out);
+// Serving teardown for cgp_engine_destroy (serve.hip).
+void serve_release_engine(cgp_engine* en);
 
 }  // namespace cgpi

@@ -32,11 +32,17 @@ void parallel_for(int64_t n, const std::function<void(int64_t, int64_t, int)>& f
     return;
   }
   t = (int)std::min<int64_t>(t, n);
+  // Dynamic chunks: per-vertex work is skewed (RMAT hubs sit at low ids), so
+  // threads take small ranges from a shared cursor instead of one slice each.
+  const int64_t chunk = std::max<int64_t>(1024, n / ((int64_t)t * 256));
+  std::atomic<int64_t> cursor{0};
   std::vector<std::thread> th;
   th.reserve(t);
   for (int i = 0; i < t; ++i) {
-    int64_t b = n * i / t, e = n * (i + 1) / t;
-    th.emplace_back([&fn, b, e, i] { fn(b, e, i); });
+    th.emplace_back([&fn, &cursor, chunk, n, i] {
+      for (int64_t b = cursor.fetch_add(chunk); b < n; b = cursor.fetch_add(chunk))
+        fn(b, std::min(n, b + chunk), i);
+    });
   }
   for (auto& x : th) x.join();
 }
@@ -365,6 +371,13 @@ std::vector<int32_t> sample_sources(const HostCSR& g, int32_t n, uint64_t seed) 
 }
 
 bool is_symmetric(const HostCSR& g) {
+  // Rows need not be sorted (COO input keeps order): sortedness is decided
+  // once per row, sorted rows are binary-searched, the others scanned.
+  std::vector<char> sorted_row((size_t)g.V);
+  parallel_for(g.V, [&](int64_t b, int64_t e, int) {
+    for (int64_t v = b; v < e; ++v)
+      sorted_row[v] = std::is_sorted(g.idx.data() + g.off[v], g.idx.data() + g.off[v + 1]) ? 1 : 0;
+  });
   std::atomic<bool> ok{true};
   parallel_for(g.V, [&](int64_t b, int64_t e, int) {
     for (int64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
@@ -372,9 +385,8 @@ bool is_symmetric(const HostCSR& g) {
         int32_t v = g.idx[k];
         const int32_t* r0 = g.idx.data() + g.off[v];
         const int32_t* r1 = g.idx.data() + g.off[v + 1];
-        // Rows need not be sorted (COO input keeps order); fall back to a scan.
-        bool found = std::is_sorted(r0, r1) ? std::binary_search(r0, r1, (int32_t)u)
-                                            : (std::find(r0, r1, (int32_t)u) != r1);
+        bool found = sorted_row[v] ? std::binary_search(r0, r1, (int32_t)u)
+                                   : (std::find(r0, r1, (int32_t)u) != r1);
         if (!found) {
           ok.store(false);
           return;

@@ -53,6 +53,9 @@ struct Cfg {
   int32_t W;            // words in use
   int32_t pull_policy;  // 0 push only, 1 BFS pull whenever BFS bits are active, 2 thresholds
   int32_t dense_t;      // popcount at which an SSSP word goes lane-per-query
+  u32 delta;            // near-far step: thr moves to far_min + delta when no pair is below
+                        // it (0: no postponing)
+  int32_t pad_;
   u64 pull_pairs_thr;   // hybrid: pull when BFS frontier (vertex,query) pairs >= this
   u64 pull_edges_thr;   // ... or when their edge-weighted count >= this
   u64 push_pairs_thr;   // back to push when pairs fall below this
@@ -72,6 +75,11 @@ struct Ctl {
   u32 pull_iters;
   u32 dense_mode;   // 1: advance uses fire-and-forget atomics, filter scans all vertices
   u32 dense_iters;
+  u32 thr;          // SSSP near-far threshold: frontier pairs with value >= thr are postponed
+                    // (INF: every pair expands, Bellman-Ford frontier)
+  u32 far_min;      // min value of the pairs postponed or improved to >= thr this iteration
+  u32 near_new;     // nonzero when this iteration produced an SSSP pair below thr
+  u32 bumps;        // threshold raises so far
   u32 pad;
   u32 bcount[2][4];           // [parity][small, medium, large items, dense items]
   u32 fetch[2][5];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high)

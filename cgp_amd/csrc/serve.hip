@@ -75,8 +75,6 @@ struct ServeState {
   std::vector<u32> wcc_host;
   // Device scratch for snapshots.
   cgpi::DevBuf d_word, d_lane, d_row, d_mask, d_rows_out, d_src;
-  // Host result buffers (pinned) recycled after cgp_serve_release_result.
-  std::vector<uint32_t*> pool;
   double clock() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -85,18 +83,29 @@ struct ServeState {
 namespace {
 
 std::mutex g_pool_mu;
-// Result buffers handed out to callers, with their owning session's pool.
-std::vector<std::pair<uint32_t*, ServeState*>> g_out;
+// Result buffers handed out to callers, with the engine whose pool takes
+// them back (null once that engine is gone: the buffer is then freed).
+std::vector<std::pair<uint32_t*, cgp_engine*>> g_out;
 
-uint32_t* take_buffer(ServeState* ss, size_t V) {
-  if (!ss->pool.empty()) {
-    uint32_t* b = ss->pool.back();
-    ss->pool.pop_back();
-    return b;
+// A pinned V-word result buffer from the engine's pool (pinning host memory
+// costs milliseconds per buffer, so buffers outlive sessions and are reused).
+uint32_t* take_buffer(cgp_engine* en, size_t V) {
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!en->result_pool.empty()) {
+      uint32_t* b = en->result_pool.back();
+      en->result_pool.pop_back();
+      return b;
+    }
   }
   uint32_t* b = nullptr;
   HIP_CHECK(hipHostMalloc((void**)&b, V * 4 + 4, hipHostMallocDefault));
   return b;
+}
+
+void hand_out(cgp_engine* en, uint32_t* b) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  g_out.push_back({b, en});
 }
 
 // Admits queued queries into free slots (within the budget) and seeds them.
@@ -125,10 +134,9 @@ void admit(cgp_engine* en, ServeState* ss) {
       c.admit_s = ss->clock();
       c.iterations = 0;
       if (ss->keep) {
-        c.result = take_buffer(ss, (size_t)V);
+        c.result = take_buffer(en, (size_t)V);
         std::memcpy(c.result, ss->wcc_host.data(), (size_t)V * 4);
-        std::lock_guard<std::mutex> lk(g_pool_mu);
-        g_out.push_back({c.result, ss});
+        hand_out(en, c.result);
       }
       c.done_s = ss->clock();
       ss->done.push_back(c);
@@ -201,7 +209,7 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
   std::vector<uint32_t*> bufs((size_t)k, nullptr);
   if (ss->keep) {
     for (int i = 0; i < k; ++i) {
-      bufs[i] = take_buffer(ss, (size_t)V);
+      bufs[i] = take_buffer(en, (size_t)V);
       HIP_CHECK(hipMemcpyAsync(bufs[i], (const u32*)ss->d_rows_out.p + (size_t)i * V, (size_t)V * 4,
                                hipMemcpyDeviceToHost, en->s));
     }
@@ -219,10 +227,7 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
     c.done_s = now;
     c.iterations = (int32_t)(ss->iter - st.start + 1);
     c.result = bufs[i];
-    if (bufs[i]) {
-      std::lock_guard<std::mutex> lk(g_pool_mu);
-      g_out.push_back({bufs[i], ss});
-    }
+    if (bufs[i]) hand_out(en, bufs[i]);
     ss->done.push_back(c);
     st.busy = false;
     --ss->busy;
@@ -233,13 +238,6 @@ void serve_free(cgp_engine* en) {
   ServeState* ss = en->serve;
   if (!ss) return;
   (void)hipStreamSynchronize(en->s);
-  {
-    std::lock_guard<std::mutex> lk(g_pool_mu);
-    // Buffers still held by the caller stay valid; they are freed on release.
-    for (auto& o : g_out)
-      if (o.second == ss) o.second = nullptr;
-  }
-  for (uint32_t* b : ss->pool) (void)hipHostFree(b);
   for (auto& c : ss->done)
     if (c.result) cgp_serve_release_result(c.result);
   ss->d_word.release();
@@ -257,6 +255,21 @@ void serve_free(cgp_engine* en) {
 }
 
 }  // namespace
+
+namespace cgpi {
+
+// Engine teardown: ends a running session, frees the pooled result buffers,
+// and detaches buffers still held by callers (freed when they are released).
+void serve_release_engine(cgp_engine* en) {
+  serve_free(en);
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (auto& o : g_out)
+    if (o.second == en) o.second = nullptr;
+  for (uint32_t* b : en->result_pool) (void)hipHostFree(b);
+  en->result_pool.clear();
+}
+
+}  // namespace cgpi
 
 extern "C" {
 
@@ -298,10 +311,13 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     cfg.W = W;
     cfg.pull_policy = 0;
     cfg.dense_t = en->opts.dense_threshold;
+    cfg.delta = (u32)en->opts.sssp_delta;
     cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
     Ctl& c = *en->h_ctl;
     std::memset(&c, 0, sizeof(Ctl));
     c.iter = 0;
+    c.thr = cfg.delta ? cfg.delta : INF;
+    c.far_min = INF;
     if (en->stamp_epoch > 0xE0000000u) {
       HIP_CHECK(hipMemsetAsync(en->mark.p, 0, (size_t)V * 4, en->s));
       en->stamp_epoch = 1;
@@ -360,9 +376,10 @@ cgp_status cgp_serve_step(cgp_engine* en, int32_t max_iterations, cgp_completion
       if (hc.finished) ss->need_restart = true;
       if (!conv.empty()) retire(en, ss, conv);
       if (max_iterations == 0 && !ss->done.empty()) break;
-      // Every query converges within V + 1 iterations of its admission.
-      if (ran > en->g->V + 3 && conv.empty() && ss->done.empty())
-        fail(CGP_EDEVICE, "cgp_serve_step: no query converged within V+3 iterations");
+      // Every query converges within V + 1 iterations of its admission per
+      // raise of the near-far threshold.
+      if (ran > ((int64_t)hc.bumps + 1) * ((int64_t)en->g->V + 3) && conv.empty() && ss->done.empty())
+        fail(CGP_EDEVICE, "cgp_serve_step: no query converged within V+3 iterations per threshold step");
     }
     int32_t nd = 0;
     while (nd < cap && !ss->done.empty()) {
@@ -388,9 +405,9 @@ void cgp_serve_release_result(uint32_t* r) {
   std::lock_guard<std::mutex> lk(g_pool_mu);
   for (size_t i = 0; i < g_out.size(); ++i) {
     if (g_out[i].first != r) continue;
-    ServeState* owner = g_out[i].second;
+    cgp_engine* owner = g_out[i].second;
     g_out.erase(g_out.begin() + (long)i);
-    if (owner) owner->pool.push_back(r);
+    if (owner) owner->result_pool.push_back(r);
     else (void)hipHostFree(r);
     return;
   }

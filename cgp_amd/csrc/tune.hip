@@ -1,6 +1,7 @@
 // tune.hip — per-graph autotuner: picks the mask-word count per pass, the
 // advance workgroup size, the degree-bin thresholds, the lane-per-query
-// threshold for SSSP words and the dense-frontier switch point by timing a
+// threshold for SSSP words, the rows in flight per step of the lane-per-query
+// kernel and the dense-frontier switch point by timing a
 // sample batch on the engine's own graph (coordinate descent, each candidate
 // timed after one warm-up run).  The reference hard-codes these (thresholds
 // 8/512 in frontier_engine.hxx:945-946, 256 threads in :2254).
@@ -74,7 +75,9 @@ extern "C" cgp_status cgp_engine_autotune(cgp_engine* en, const cgp_query* sampl
       consider([](cgp_engine_opts& o, int v) { o.block_size = v; }, {128, 256, 512});
       consider([](cgp_engine_opts& o, int v) { o.bin_small = v; }, {16, 32, 64, 128, 256});
       consider([](cgp_engine_opts& o, int v) { o.bin_large = std::max(v, o.bin_small); }, {512, 2048, 8192});
-      consider([](cgp_engine_opts& o, int v) { o.dense_threshold = v; }, {8, 16, 32, 65});
+      consider([](cgp_engine_opts& o, int v) { o.dense_threshold = v; }, {1, 2, 4, 8, 16, 32, 65});
+      consider([](cgp_engine_opts& o, int v) { o.dense_unroll = v; }, {4, 8});
+      consider([](cgp_engine_opts& o, int v) { o.sssp_delta = v; }, {0, 32, 128, 512, 2048, 8192});
       consider([](cgp_engine_opts& o, int v) { o.frontier_dense_div = v; }, {2, 8, 32});
     } catch (...) {
       en->opts = start;

@@ -87,6 +87,12 @@ typedef struct {
                                   lane-per-query (one wave per edge row); 0 = 16 */
   int32_t frontier_dense_div;  /* dense frontier mode (fire-and-forget atomics, filter scans all
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
+  int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4 or 8
+                                  (fewer registers, more waves vs more rows per wave); 0 = 8 */
+  int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
+                                  wait; the threshold moves to the smallest waiting value + delta
+                                  when nothing below it is left.  0 = no waiting (Bellman-Ford
+                                  frontier).  Results are identical either way. */
 } cgp_engine_opts;
 
 /* Per-run report; replaces run_result_t / iteration_profile_t

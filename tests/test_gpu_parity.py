@@ -127,6 +127,48 @@ def test_mixed_uneven_batch(gpu, hip_graph, profile):
         assert st["advance_bytes"] > 0 and st["edges_scanned"] > 0
 
 
+@pytest.mark.parametrize("unroll", [4, 8])
+@pytest.mark.parametrize("dense_t", [1, 4, 65])
+@pytest.mark.parametrize("delta", [0, 1, 100])
+def test_mixed_lane_per_query_variants(gpu, unroll, dense_t, delta):
+    # dense_threshold 1 sends every SSSP word to the lane-per-query kernel,
+    # 65 none; both row-unroll depths; near-far thresholds off, at the
+    # smallest step and at a mid step.
+    say("mixed unroll", unroll, "dense_t", dense_t, "delta", delta)
+    h, g, og = rmat(12, 16, 6, 1, 255)
+    src = h.sample_sources(240, seed=5)
+    kinds = [BFS] * 100 + [SSSP] * 140 + [WCC] * 2
+    sources = list(src) + [0] * 2
+    en = cgp_amd.Engine(g, dense_unroll=unroll, dense_threshold=dense_t, sssp_delta=delta)
+    out, st = en.run_batch(kinds, sources)
+    assert np.array_equal(out, oracle_rows(og, kinds, sources))
+
+
+@pytest.mark.parametrize("delta", [1, 7, 3000])
+@pytest.mark.parametrize("nq", [1, 64])
+def test_near_far_sssp_rmat(gpu, delta, nq):
+    # Near-far postponing on one-word and single-query batches (R < 64 rows).
+    say("near-far rmat delta", delta, "nq", nq)
+    h, g, og = rmat(13, 8, 9, 1, 255)
+    src = h.sample_sources(nq, seed=delta)
+    en = cgp_amd.Engine(g, sssp_delta=delta)
+    out, _ = en.run_batch([SSSP] * nq, src)
+    assert np.array_equal(out, oracle_rows(og, [1] * nq, src))
+
+
+def test_near_far_grid_long_paths(gpu):
+    # Long shortest paths (hundreds of hops) with a small step: many threshold
+    # raises, sparse frontiers; against the oracle and the Bellman-Ford run.
+    h = cgp_amd.HostCSR.grid(64, 64, p_extra=0.0, seed=3, wmin=1, wmax=1000)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = [0, 4095, 2080, 17]
+    ref = cgp_amd.Engine(g).run_batch([SSSP] * 4, src)[0]
+    out, st = cgp_amd.Engine(g, sssp_delta=50).run_batch([SSSP] * 4, src)
+    assert np.array_equal(out, ref)
+    assert np.array_equal(out, oracle_rows(og, [1] * 4, src))
+
+
 def test_alone_equals_in_the_mix(gpu):
     # Each query's result is identical run alone and inside a mixed batch.
     h, g, og = rmat(11, 16, 8, 1, 100)

@@ -43,14 +43,14 @@ def test_replenish_golden_two_slots(gpu):
     assert max(done[2]["admit_s"], done[3]["admit_s"]) >= min(done[0]["done_s"], done[1]["done_s"])
 
 
-@pytest.mark.parametrize("slots", [4, 64, 130])
-def test_staggered_admission_matches_oracle(gpu, slots):
+@pytest.mark.parametrize("slots,delta", [(4, 0), (64, 0), (130, 0), (64, 40)])
+def test_staggered_admission_matches_oracle(gpu, slots, delta):
     h = cgp_amd.HostCSR.rmat(12, 16, seed=21, wmin=1, wmax=200)
     g = cgp_amd.Graph.from_host(h)
     og = O.G.of(h)
     src = h.sample_sources(90, seed=4)
     kinds = [(BFS, SSSP, BFS)[i % 3] for i in range(90)]
-    en = cgp_amd.Engine(g)
+    en = cgp_amd.Engine(g, sssp_delta=delta)
     en.serve_start(max_slots=slots)
     done = {}
     nxt = 0
