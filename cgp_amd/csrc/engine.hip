@@ -131,8 +131,9 @@ __device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
 // the near-far statistics of its SSSP improvements (flushed whenever the
 // threshold is active).
 struct Tally {
-  u64 items = 0, edges = 0, wedges = 0, bfs = 0, sssp = 0, nnew = 0, imp = 0;
-  u64 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
+  u32 items = 0, edges = 0, wedges = 0, bfs = 0, nnew = 0, imp = 0;
+  u32 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
+  u64 sssp = 0;  // (edge, query) pairs: can pass 2^32 per lane
   u32 fmin = INF;  // min value of pairs postponed or improved to >= thr
   u32 nnear = 0;   // improvements to a value < thr
 };
@@ -359,7 +360,9 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
   const u32 d = work ? (u32)(P.off[u + 1] - e0) : 0u;
   if (work) ++t.items;
   const u32 incl = wave_incl_scan(d);
-  const u32 total = (u32)__shfl((int)incl, 63, 64);
+  // Wave-uniform (scalar) so the loops below branch on the scalar unit and
+  // the per-step row indices stay in scalar registers.
+  const u32 total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
   for (u32 win = 0; win < total; win += 64) {
     const u32 f = win + (u32)lane;
     const int j = wave_upper_bound(incl, f);
@@ -412,8 +415,8 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   u64 sm = have ? (P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) : 0ull;
   const bool inrow = lane < R;
   // Lane offset inside a value row; every lane reads a valid address of the
-  // (always valid) row, inactive lanes just discard what they read, so row
-  // indices stay wave-uniform.
+  // (always valid) row whether or not its pair is active, so the loads issue
+  // before activity is known and row indices stay wave-uniform.
   const int lq = lane & (R - 1);
   if (thr != INF) {
     // Near-far split of the wave's items: lane q reads value q of each
@@ -466,9 +469,20 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
     t.sssp += (u64)d * (u64)__popcll(sm);
   }
   const u32 incl = wave_incl_scan(d);
-  const u32 total = (u32)__shfl((int)incl, 63, 64);
+  // Wave-uniform (scalar) so the loops below branch on the scalar unit and
+  // the per-step row indices stay in scalar registers.
+  const u32 total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
   const u32 smlo = (u32)sm, smhi = (u32)(sm >> 32);
+  const u64 item_row = (u64)((u32)w * (u32)P.V + (u32)u);  // source row of the lane's item
+  const u64 row_lanes = R >= 64 ? ~0ull : ((1ull << R) - 1ull);
+  const u64 vals_base = (u64)P.vals;
+  const int rsh = rshift + 2;  // row index -> byte offset
   for (u32 win = 0; win < total; win += 64) {
+    // Lane f takes flattened edge win+f and computes, once per window, the
+    // byte addresses of its target and source rows, its weight and its
+    // item's active-query mask; each step then reads them back with
+    // independent uniform-index readlanes (no per-step scalar arithmetic).
+    // Lanes past the end point at row 0 with an empty mask.
     const u32 f = win + (u32)lane;
     const int j = wave_upper_bound(incl, f);
     const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
@@ -477,77 +491,77 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
     const int32_t myv = ok ? P.idx[ej] : 0;
     const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : 0u;
     const int myword = __shfl(w, j, 64);
+    const u64 my_vaddr = vals_base + ((u64)((u32)myword * (u32)P.V + (u32)myv) << rsh);
+    const u64 my_uaddr = vals_base + ((u64)(u32)__shfl((int)(u32)item_row, j, 64) << rsh);
+    // Shuffles run with every lane active (a lane past the end may still be
+    // the source of another lane's item), then the select.
+    const u32 am_lo = (u32)__shfl((int)smlo, j, 64), am_hi = (u32)__shfl((int)smhi, j, 64);
+    const u64 my_am = ok ? ((((u64)am_hi << 32) | am_lo) & row_lanes) : 0ull;
     const int cnt = (int)min(64u, total - win);
-    // Per-step state: target row (w*V + v, scaled by R at use), weight,
-    // loaded values, activity.
-    u32 va[U];
+    // Per-step state: uniform row pointers, weight and active mask; per
+    // lane the two loaded values.
+    u64 vr[U], am[U];
     u32 cand[U], cur[U], du[U];
-    bool act[U];
     auto issue = [&](int k0) {
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        const int jj = min(k0 + k, cnt - 1);
-        const int32_t vk = __builtin_amdgcn_readlane(myv, jj);
-        const int it = __builtin_amdgcn_readlane(j, jj);
-        const int32_t uk = __builtin_amdgcn_readlane(u, it);
-        const int wk = __builtin_amdgcn_readlane(w, it);
-        const u64 smk = ((u64)(u32)__builtin_amdgcn_readlane((int)smhi, it) << 32) |
-                        (u32)__builtin_amdgcn_readlane((int)smlo, it);
-        act[k] = (k0 + k < cnt) && inrow && ((smk >> lane) & 1ull);
-        va[k] = (u32)wk * (u32)P.V + (u32)vk;
-        const u32 ur = (u32)wk * (u32)P.V + (u32)uk;
+        const int jj = k0 + k;  // < 64: k0 < cnt <= 64 and U divides 64
+        vr[k] = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_vaddr >> 32), jj) << 32) |
+                (u32)__builtin_amdgcn_readlane((int)(u32)my_vaddr, jj);
+        const u64 ua = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_uaddr >> 32), jj) << 32) |
+                       (u32)__builtin_amdgcn_readlane((int)(u32)my_uaddr, jj);
         cand[k] = (u32)__builtin_amdgcn_readlane((int)myw, jj);
-        const gu32* urow = uniform_gptr(P.vals + ((size_t)ur << rshift));
-        const gu32* vrw = uniform_gptr(P.vals + ((size_t)va[k] << rshift));
-        du[k] = urow[(unsigned)lq];
-        cur[k] = vrw[(unsigned)lq];
+        am[k] = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_am >> 32), jj) << 32) |
+                (u32)__builtin_amdgcn_readlane((int)(u32)my_am, jj);
+        du[k] = ((const gu32*)ua)[(unsigned)lq];
+        cur[k] = ((const gu32*)vr[k])[(unsigned)lq];
       }
     };
     issue(0);
     for (int k0 = 0; k0 < cnt; k0 += U) {
-      u32 va_c[U];
-      u32 cand_c[U];
-      bool imp[U];
+      u64 vr_c[U], mk[U];
+      u32 c_c[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        u32 c = du[k] + cand[k];
-        if (c < du[k]) c = INF;
-        imp[k] = act[k] && c < cur[k];
-        cand_c[k] = c;
-        va_c[k] = va[k];
+        // Saturating add: a path past CGP_INF stays unreachable.
+        const u32 c = __builtin_elementwise_add_sat(du[k], cand[k]);
+        mk[k] = __ballot(c < cur[k]) & am[k];  // lanes whose value improves
+        c_c[k] = c;
+        vr_c[k] = vr[k];
       }
-      int32_t vv_c[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) vv_c[k] = __builtin_amdgcn_readlane(myv, min(k0 + k, cnt - 1));
       if (k0 + U < cnt) issue(k0 + U);
-      u64 mine = 0;
+      u32 mine_lo = 0, mine_hi = 0;
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        if (imp[k]) {
-          __hip_atomic_fetch_min(uniform_gptr(P.vals + ((size_t)va_c[k] << rshift)) + (unsigned)lq, cand_c[k],
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          note_improved(t, cand_c[k], thr);
+        const u64 m = mk[k];
+        if (m == 0) continue;
+        if ((m >> lane) & 1ull) {
+          __hip_atomic_fetch_min((gu32*)vr_c[k] + (unsigned)lq, c_c[k], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          note_improved(t, c_c[k], thr);
+          ++t.imp;
         }
-        const u64 m = __ballot(imp[k]);
         if (!SPARSE) {
-          mine = (lane == ((k0 + k) & 63)) ? m : mine;
-        } else if (m) {
+          const bool own = lane == k0 + k;
+          mine_lo = own ? (u32)m : mine_lo;
+          mine_hi = own ? (u32)(m >> 32) : mine_hi;
+        } else {
+          const int32_t vk = __builtin_amdgcn_readlane(myv, k0 + k);
           bool claimed = false;
           if (lane == 0) {
-            const int wk = __builtin_amdgcn_readlane(myword, min(k0 + k, cnt - 1));
-            if (atomicOr(&P.next[(size_t)vv_c[k] * W + wk], m) == 0)
-              claimed = claim_vertex(P.mark, vv_c[k], stamp);
+            ++t.nnew;
+            const int wk = __builtin_amdgcn_readlane(myword, k0 + k);
+            if (atomicOr(&P.next[(size_t)vk * W + wk], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
           }
-          wave_append<int32_t>(claimed, vv_c[k], &P.ctl->tcount, P.touched);
-        }
-        if (m) {
-          t.imp += imp[k] ? 1 : 0;
-          if (lane == 0) ++t.nnew;
+          wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
         }
       }
       // Dense mode: lane k0+k owns edge k0+k's next-word update, one masked
       // instruction for the whole step.
-      if (!SPARSE && mine) atomicOr(&P.next[(size_t)myv * W + myword], mine);
+      if (!SPARSE && (mine_lo | mine_hi)) {
+        ++t.nnew;
+        atomicOr(&P.next[(size_t)myv * W + myword], ((u64)mine_hi << 32) | mine_lo);
+      }
     }
   }
 }
@@ -1183,17 +1197,21 @@ __global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, int32_t V
 // ============================================================== host side
 namespace {
 
-// Both frontier-mode instantiations of the dense-SSSP advance (the one not
-// matching the device-selected mode exits at once), U = rows per step.
-template <int W, int BS>
-void launch_dense(hipStream_t s, dim3 grid, const KP& P, bool u4) {
-  if (u4) {
-    hipLaunchKernelGGL((k_adv_dense<W, BS, false, 4>), grid, dim3(BS), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, BS, true, 4>), grid, dim3(BS), 0, s, P);
-  } else {
-    hipLaunchKernelGGL((k_adv_dense<W, BS, false, 8>), grid, dim3(BS), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, BS, true, 8>), grid, dim3(BS), 0, s, P);
-  }
+// Both frontier-mode instantiations of the lane-per-query advance (the one
+// not matching the device-selected mode exits at once), U = rows per step.
+// It is wave-level work, so it always runs 256-thread workgroups, 8 per CU.
+template <int W, int U>
+void launch_dense_u(hipStream_t s, dim3 grid, const KP& P) {
+  hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U>), grid, dim3(BLOCK), 0, s, P);
+  hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U>), grid, dim3(BLOCK), 0, s, P);
+}
+
+template <int W>
+void launch_dense(hipStream_t s, int cus, const KP& P, int unroll) {
+  const dim3 grid((unsigned)cus * 8u);
+  if (unroll == 4) launch_dense_u<W, 4>(s, grid, P);
+  else if (unroll == 16) launch_dense_u<W, 16>(s, grid, P);
+  else launch_dense_u<W, 8>(s, grid, P);
 }
 
 // One iteration: advance then filter (both time themselves on the device clock).
@@ -1201,16 +1219,13 @@ template <int W>
 void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
   const int bs = en->opts.block_size;
-  const bool u4 = en->opts.dense_unroll == 4;
   const dim3 grid(en->grid_advance());
+  launch_dense<W>(s, en->cus, P, en->opts.dense_unroll);
   if (bs == 128) {
-    launch_dense<W, 128>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
   } else if (bs == 512) {
-    launch_dense<W, 512>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 512>), grid, dim3(512), 0, s, P);
   } else {
-    launch_dense<W, 256>(s, grid, P, u4);
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, s, P);
   }
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
@@ -1390,10 +1405,12 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     dout = (u32*)en->outbuf.p;
   }
 
-  // Passes of at most max_words*64 slots; BFS slots first, then SSSP.
+  // Passes of at most max_words*64 slots; SSSP slots first, then BFS, so the
+  // SSSP slots fill the fewest words (value rows of BFS-only words are never
+  // read by the advance).
   const int slots_max = std::max(1, std::min(en->opts.max_words, MAXW)) * 64;
-  std::vector<int32_t> order(bfs);
-  order.insert(order.end(), sssp.begin(), sssp.end());
+  std::vector<int32_t> order(sssp);
+  order.insert(order.end(), bfs.begin(), bfs.end());
   std::vector<double> prof_sum(NPROF, 0.0);
   for (size_t p0 = 0; p0 < order.size(); p0 += slots_max) {
     const int ns = (int)std::min<size_t>(slots_max, order.size() - p0);
@@ -1688,7 +1705,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->bin_small = 64;
   o->bin_large = 2048;
   o->use_hip_graph = 1;
-  o->dense_threshold = 16;
+  o->dense_threshold = 8;
   o->frontier_dense_div = 8;
   o->dense_unroll = 8;
   o->sssp_delta = 0;
@@ -1727,8 +1744,8 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.dense_threshold <= 0) en->opts.dense_threshold = d.dense_threshold;
     if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
     if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
-    if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8)
-      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4 or 8");
+    if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8 && en->opts.dense_unroll != 16)
+      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4, 8 or 16");
     if (en->opts.sssp_delta < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= 0");
     en->cus = device_cus();
     {

@@ -76,7 +76,7 @@ extern "C" cgp_status cgp_engine_autotune(cgp_engine* en, const cgp_query* sampl
       consider([](cgp_engine_opts& o, int v) { o.bin_small = v; }, {16, 32, 64, 128, 256});
       consider([](cgp_engine_opts& o, int v) { o.bin_large = std::max(v, o.bin_small); }, {512, 2048, 8192});
       consider([](cgp_engine_opts& o, int v) { o.dense_threshold = v; }, {1, 2, 4, 8, 16, 32, 65});
-      consider([](cgp_engine_opts& o, int v) { o.dense_unroll = v; }, {4, 8});
+      consider([](cgp_engine_opts& o, int v) { o.dense_unroll = v; }, {4, 8, 16});
       consider([](cgp_engine_opts& o, int v) { o.sssp_delta = v; }, {0, 32, 128, 512, 2048, 8192});
       consider([](cgp_engine_opts& o, int v) { o.frontier_dense_div = v; }, {2, 8, 32});
     } catch (...) {

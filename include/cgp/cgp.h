@@ -84,11 +84,11 @@ typedef struct {
                                   bit 1: device work counters -> advance_bytes */
   uint64_t hbm_budget_bytes;   /* serving: cap on per-query value memory; 0 = 64 GiB */
   int32_t dense_threshold;     /* active SSSP queries in a mask word at which the word is relaxed
-                                  lane-per-query (one wave per edge row); 0 = 16 */
+                                  lane-per-query (one wave per edge row); 0 = 8 */
   int32_t frontier_dense_div;  /* dense frontier mode (fire-and-forget atomics, filter scans all
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
-  int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4 or 8
-                                  (fewer registers, more waves vs more rows per wave); 0 = 8 */
+  int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4, 8
+                                  or 16 (fewer registers, more waves vs more rows per wave); 0 = 8 */
   int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
                                   wait; the threshold moves to the smallest waiting value + delta
                                   when nothing below it is left.  0 = no waiting (Bellman-Ford

@@ -127,7 +127,7 @@ def test_mixed_uneven_batch(gpu, hip_graph, profile):
         assert st["advance_bytes"] > 0 and st["edges_scanned"] > 0
 
 
-@pytest.mark.parametrize("unroll", [4, 8])
+@pytest.mark.parametrize("unroll", [4, 8, 16])
 @pytest.mark.parametrize("dense_t", [1, 4, 65])
 @pytest.mark.parametrize("delta", [0, 1, 100])
 def test_mixed_lane_per_query_variants(gpu, unroll, dense_t, delta):
