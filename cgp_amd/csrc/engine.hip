@@ -109,6 +109,18 @@ __device__ __forceinline__ size_t vrow(int32_t V, int w, int32_t v, int rshift) 
   return ((size_t)w * (size_t)V + (size_t)v) << rshift;
 }
 
+// Visited and next-frontier words of a vertex form one record of 2W words
+// (W visited words, then W next words), so an edge that tests visited and
+// next touches one record; P.visited and P.next share the base.
+template <int W>
+__device__ __forceinline__ size_t vis_at(int32_t v, int w) {
+  return (size_t)v * (2 * W) + (size_t)w;
+}
+template <int W>
+__device__ __forceinline__ size_t nxt_at(int32_t v, int w) {
+  return (size_t)v * (2 * W) + W + (size_t)w;
+}
+
 __device__ __forceinline__ void prof_add(const KP& P, int slot, u64 x) {
   x = wave_sum64(x);
   if (lane_id() == 0 && x) atomicAdd(&P.ctl->prof[slot], x);
@@ -153,13 +165,13 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
                                            const u64 (&bm)[W], const u64 (&sm)[W], u32 stamp,
                                            int rshift, u32 thr, Tally& t) {
   bool first = false;
-  const size_t vb = (size_t)v * W;
-  // All W visited / next words of v in one go (each array's W words are one
-  // contiguous sector), issued before any of them is used.
+  const size_t vb = vis_at<W>(v, 0);
+  // All W visited and W next words of v in one go (one contiguous record),
+  // issued before any of them is used.
   u64 nx[W], vis[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    nx[w] = P.next[vb + w];
+    nx[w] = P.next[vb + W + w];
     vis[w] = P.visited[vb + w];
   }
 #pragma unroll
@@ -204,9 +216,9 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
     if (add & ~nx[w]) {
       ++t.nnew;
       if (SPARSE) {
-        if (atomicOr(&P.next[vb + w], add) == 0) first = true;
+        if (atomicOr(&P.next[vb + W + w], add) == 0) first = true;
       } else {
-        atomicOr(&P.next[vb + w], add);
+        atomicOr(&P.next[vb + W + w], add);
       }
     }
   }
@@ -269,9 +281,9 @@ __device__ __forceinline__ bool postpone(const KP& P, int32_t u, const u64 (&far
     if (!far[w]) continue;
     ++t.nnew;
     if (SPARSE) {
-      if (atomicOr(&P.next[(size_t)u * W + w], far[w]) == 0) first = true;
+      if (atomicOr(&P.next[nxt_at<W>(u, w)], far[w]) == 0) first = true;
     } else {
-      atomicOr(&P.next[(size_t)u * W + w], far[w]);
+      atomicOr(&P.next[nxt_at<W>(u, w)], far[w]);
     }
   }
   return SPARSE && first && claim_vertex(P.mark, u, stamp);
@@ -418,6 +430,15 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   // (always valid) row whether or not its pair is active, so the loads issue
   // before activity is known and row indices stay wave-uniform.
   const int lq = lane & (R - 1);
+  // Bit w: this lane holds an SSSP slot of word w.  Other lanes read the
+  // row's first value instead of their own, so a word whose SSSP slots fill
+  // only part of the row fetches only the lines they occupy (SSSP slots are
+  // packed from lane 0).
+  u32 lane_words = 0;
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) lane_words |= (u32)((P.cfg->val_bits[ww] >> lq) & 1ull) << ww;
+  if (!inrow) lane_words = 0;
+  const u32 lq4 = (u32)lq * 4u;
   if (thr != INF) {
     // Near-far split of the wave's items: lane q reads value q of each
     // item's source row (8 rows in flight); pairs at or above thr are
@@ -451,9 +472,9 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
           if (lane == 0) {
             ++t.nnew;
             if (SPARSE) {
-              if (atomicOr(&P.next[(size_t)uk * W + wk], fm) == 0) claimed = claim_vertex(P.mark, uk, stamp);
+              if (atomicOr(&P.next[nxt_at<W>(uk, wk)], fm) == 0) claimed = claim_vertex(P.mark, uk, stamp);
             } else {
-              atomicOr(&P.next[(size_t)uk * W + wk], fm);
+              atomicOr(&P.next[nxt_at<W>(uk, wk)], fm);
             }
           }
           if (SPARSE) wave_append<int32_t>(claimed, uk, &P.ctl->tcount, P.touched);
@@ -513,8 +534,10 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
         cand[k] = (u32)__builtin_amdgcn_readlane((int)myw, jj);
         am[k] = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_am >> 32), jj) << 32) |
                 (u32)__builtin_amdgcn_readlane((int)(u32)my_am, jj);
-        du[k] = ((const gu32*)ua)[(unsigned)lq];
-        cur[k] = ((const gu32*)vr[k])[(unsigned)lq];
+        const u32 wk = (u32)__builtin_amdgcn_readlane(myword, jj);
+        const u32 off = ((lane_words >> wk) & 1u) ? lq4 : 0u;
+        du[k] = *(const gu32*)(ua + off);
+        cur[k] = *(const gu32*)(vr[k] + off);
       }
     };
     issue(0);
@@ -551,7 +574,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
           if (lane == 0) {
             ++t.nnew;
             const int wk = __builtin_amdgcn_readlane(myword, k0 + k);
-            if (atomicOr(&P.next[(size_t)vk * W + wk], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
+            if (atomicOr(&P.next[nxt_at<W>(vk, wk)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
           }
           wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
         }
@@ -560,7 +583,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
       // instruction for the whole step.
       if (!SPARSE && (mine_lo | mine_hi)) {
         ++t.nnew;
-        atomicOr(&P.next[(size_t)myv * W + myword], ((u64)mine_hi << 32) | mine_lo);
+        atomicOr(&P.next[nxt_at<W>(myv, myword)], ((u64)mine_hi << 32) | mine_lo);
       }
     }
   }
@@ -579,7 +602,7 @@ __device__ __forceinline__ bool pull_vertex(const KP& P, int32_t v, const u64 (&
   bool any = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    tgt[w] = bb[w] & ~P.visited[(size_t)v * W + w];
+    tgt[w] = bb[w] & ~P.visited[vis_at<W>(v, w)];
     acc[w] = 0;
     any |= tgt[w] != 0;
   }
@@ -594,7 +617,7 @@ __device__ __forceinline__ bool pull_vertex(const KP& P, int32_t v, const u64 (&
     for (int w = 0; w < W; ++w) {
       if (tgt[w] & ~acc[w]) {
         ++t.pwords;
-        acc[w] |= P.visited[(size_t)u * W + w] & tgt[w];
+        acc[w] |= P.visited[vis_at<W>(u, w)] & tgt[w];
         done &= (acc[w] == tgt[w]);
       }
     }
@@ -606,9 +629,9 @@ __device__ __forceinline__ bool pull_vertex(const KP& P, int32_t v, const u64 (&
     if (acc[w]) {
       ++t.nnew;
       if (SPARSE) {
-        if (atomicOr(&P.next[(size_t)v * W + w], acc[w]) == 0) first = true;
+        if (atomicOr(&P.next[nxt_at<W>(v, w)], acc[w]) == 0) first = true;
       } else {
-        atomicOr(&P.next[(size_t)v * W + w], acc[w]);
+        atomicOr(&P.next[nxt_at<W>(v, w)], acc[w]);
       }
     }
   }
@@ -627,7 +650,7 @@ __device__ __forceinline__ bool pull_vertex_wave(const KP& P, int32_t v, const u
   bool any = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    tgt[w] = bb[w] & ~P.visited[(size_t)v * W + w];
+    tgt[w] = bb[w] & ~P.visited[vis_at<W>(v, w)];
     acc[w] = 0;
     any |= tgt[w] != 0;
   }
@@ -641,7 +664,7 @@ __device__ __forceinline__ bool pull_vertex_wave(const KP& P, int32_t v, const u
     bool done = true;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const u64 a = ok ? (P.visited[(size_t)u * W + w] & tgt[w]) : 0ull;
+      const u64 a = ok ? (P.visited[vis_at<W>(u, w)] & tgt[w]) : 0ull;
       acc[w] |= wave_or64(a);
       done &= acc[w] == tgt[w];
     }
@@ -658,9 +681,9 @@ __device__ __forceinline__ bool pull_vertex_wave(const KP& P, int32_t v, const u
       if (acc[w]) {
         ++t.nnew;
         if (SPARSE) {
-          if (atomicOr(&P.next[(size_t)v * W + w], acc[w]) == 0) first = true;
+          if (atomicOr(&P.next[nxt_at<W>(v, w)], acc[w]) == 0) first = true;
         } else {
-          atomicOr(&P.next[(size_t)v * W + w], acc[w]);
+          atomicOr(&P.next[nxt_at<W>(v, w)], acc[w]);
         }
       }
     }
@@ -852,7 +875,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, i
   if (s < nslots && slot_src[s] >= 0) {
     v = slot_src[s];
     const int w = s >> 6, q = s & 63;
-    atomicOr(&P.next[(size_t)v * W + w], 1ull << q);
+    atomicOr(&P.next[nxt_at<W>(v, w)], 1ull << q);
     if ((P.cfg->val_bits[w] >> q) & 1ull) {
       P.vals[vrow(P.V, w, v, P.cfg->rshift) + q] = 0u;
       atomicOr(&C->near_new, 1u);  // a source at value 0 is below any threshold
@@ -898,7 +921,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     bool anyw = false;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      nwv[w] = valid ? P.next[(size_t)v * W + w] : 0ull;
+      nwv[w] = valid ? P.next[nxt_at<W>(v, w)] : 0ull;
       anyw |= nwv[w] != 0;
     }
     const int32_t deg = anyw ? P.off[v + 1] - P.off[v] : 0;
@@ -909,11 +932,11 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     for (int w = 0; w < W; ++w) {
       const u64 nw = nwv[w];
       if (anyw) {
-        if (nw) P.next[(size_t)v * W + w] = 0;
+        if (nw) P.next[nxt_at<W>(v, w)] = 0;
         P.front[(size_t)v * W + w] = nw;
       }
       nb[w] = nw & bb[w];
-      if (nb[w]) P.visited[(size_t)v * W + w] |= nb[w];
+      if (nb[w]) P.visited[vis_at<W>(v, w)] |= nb[w];
       const u64 s = nw & vb[w];
       if (s && __popcll(s) >= dense_t) dmask |= 1u << w;
       else if (s) need_bin = true;
@@ -1488,11 +1511,10 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     HIP_CHECK(hipMemcpyAsync(en->slot_src.p, h_src, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->qmap.p, h_qmap, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
     fill_u32(en->s, en->vals.p, (size_t)W * V * R, INF, en->cus);
-    if (nb > 0) fill_u32(en->s, en->visited.p, (size_t)V * W * 2, 0u, en->cus);
-    if (en->next_dirty) {
-      fill_u32(en->s, en->next.p, (size_t)V * W * 2, 0u, en->cus);
-      en->next_dirty = false;
-    }
+    // Visited + next records (2W words per vertex) start clear; the layout
+    // depends on W, so the whole region is cleared for every pass.
+    fill_u32(en->s, en->visited.p, (size_t)V * W * 4, 0u, en->cus);
+    en->next_dirty = false;
     dispatch_seed(en, W, P, (const int32_t*)en->slot_src.p, W * 64);
     en->tick(CAT_INIT, false);
     run_iterations(en, W);
@@ -1771,7 +1793,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   if (en->s) (void)hipStreamSynchronize(en->s);
   cgpi::serve_release_engine(en);
   en->drop_graphs();
-  DevBuf* bufs[] = {&en->visited, &en->next, &en->front, &en->mark, &en->vals, &en->touched,
+  DevBuf* bufs[] = {&en->visited, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
                     &en->large[1], &en->dense[0], &en->dense[1], &en->parent, &en->wcc_rows,
                     &en->outbuf, &en->qmap, &en->slot_src, &en->iters};

@@ -101,7 +101,7 @@ struct cgp_engine {
   hipStream_t s = nullptr;
   int cus = 1;
   int wcap = 0;            // words allocated in visited/next/front/dense
-  cgpi::DevBuf visited, next, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];
+  cgpi::DevBuf visited, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];  // visited: visited + next records
   cgpi::DevBuf iters;  // per-iteration record ring (ITER_RING entries)
   std::vector<cgpk::IterRec> report;  // records of the last batch run, all passes
   cgpi::DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
@@ -148,8 +148,9 @@ struct cgp_engine {
     P.pull_hi = g->pull_hi;
     P.n_pull_lo = g->n_pull_lo;
     P.n_pull_hi = g->n_pull_hi;
+    // One buffer of per-vertex records: W visited words then W next words.
     P.visited = (u64*)visited.p;
-    P.next = (u64*)next.p;
+    P.next = (u64*)visited.p;
     P.front = (u64*)front.p;
     P.mark = (u32*)mark.p;
     P.vals = (u32*)vals.p;
@@ -190,10 +191,9 @@ struct cgp_engine {
     const size_t V = (size_t)g->V, E = (size_t)g->E;
     bool moved = false;
     if (W > wcap) {
-      moved |= visited.ensure(V * W * 8);
-      moved |= next.ensure(V * W * 8);
+      moved |= visited.ensure(V * W * 16);  // visited + next records
       moved |= front.ensure(V * W * 8);
-      HIP_CHECK(hipMemsetAsync(next.p, 0, V * W * 8, s));
+      HIP_CHECK(hipMemsetAsync(visited.p, 0, V * W * 16, s));
       for (int k = 0; k < 2; ++k) moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
       wcap = W;
     }

@@ -4,7 +4,10 @@
 // Layout in HBM (V vertices, W mask words per vertex, R lanes per value row):
 //   off[V+1], idx[E] int32; wt[E] uint32 (absent = unit weights); rows[E] int32
 //     (source vertex of each CSR slot, built once per graph for edge-parallel WCC)
-//   visited/next/front  uint64[V*W]   vertex-major, word w of vertex v at v*W+w
+//   visited + next  uint64[V*2W]       one record per vertex: visited word w at
+//                                      v*2W+w, next-frontier word w at v*2W+W+w
+//                                      (an edge's BFS test reads one record)
+//   front  uint64[V*W]                 frontier word w of vertex v at v*W+w
 //   vals  uint32[W*V*R]                value of slot (w, q) at vertex v at
 //                                      ((w*V + v) << log2 R) + q: the R query
 //                                      values of one word of one vertex are one

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visite
       vals[at] = INF;
     }
     for (int w = 0; w < W; ++w)
-      if (clear_mask[w]) visited[(size_t)v * W + w] &= ~clear_mask[w];
+      if (clear_mask[w]) visited[(size_t)v * (2 * W) + w] &= ~clear_mask[w];  // visited half of the record
   }
 }
 
@@ -300,11 +300,8 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     ss->d_src.ensure((size_t)ss->nslots * 4);
     // Clean slot state: every value row INF, no visited bits, no pending next bits.
     fill_u32(en->s, en->vals.p, (size_t)W * V * 64, INF, en->cus);
-    fill_u32(en->s, en->visited.p, (size_t)V * W * 2, 0u, en->cus);
-    if (en->next_dirty) {
-      fill_u32(en->s, en->next.p, (size_t)V * W * 2, 0u, en->cus);
-      en->next_dirty = false;
-    }
+    fill_u32(en->s, en->visited.p, (size_t)V * W * 4, 0u, en->cus);  // visited + next records
+    en->next_dirty = false;
     Cfg& cfg = *en->h_cfg;
     std::memset(&cfg, 0, sizeof(Cfg));
     cfg.rshift = 6;
