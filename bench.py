@@ -192,14 +192,15 @@ def serve_bench(args):
             while nxt < nq and arrivals[nxt] <= now:
                 en.serve_submit(kinds[nxt], src[nxt], nxt, t_base + arrivals[nxt])
                 nxt += 1
-            if en.serve_pending() == (0, 0) and nxt < nq:
-                time.sleep(min(0.001, max(0.0, arrivals[nxt] - now)))
-                continue
-            for c in en.serve_step(max_iterations=1, fetch=check):
+            # One iteration (an idle engine only finishes its result copies).
+            got = en.serve_step(max_iterations=1, fetch=check)
+            for c in got:
                 if check:
                     same = same and bool(np.array_equal(c["result"], batch_rows[c["ticket"]]))
                     c["result"] = None
                 done[c["ticket"]] = c
+            if not got and en.serve_pending() == (0, 0) and nxt < nq:
+                time.sleep(min(0.001, max(0.0, arrivals[nxt] - now)))
         en.serve_stop()
         return done, same
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -192,8 +193,10 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
         u32 du[4], cur[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
+          // Unused slots of the batch re-read the first slot's values (same
+          // lines), never a line no active query needs.
           a[i] = s != 0;
-          q[i] = a[i] ? __ffsll((long long)s) - 1 : 0;
+          q[i] = a[i] ? __ffsll((long long)s) - 1 : q[0];
           if (a[i]) s &= s - 1;
           du[i] = P.vals[ub + q[i]];
           cur[i] = P.vals[vr + q[i]];
@@ -251,7 +254,7 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           a[i] = x != 0;
-          q[i] = a[i] ? __ffsll((long long)x) - 1 : 0;
+          q[i] = a[i] ? __ffsll((long long)x) - 1 : q[0];
           if (a[i]) x &= x - 1;
           d[i] = P.vals[ub + q[i]];
         }
@@ -1659,6 +1662,11 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
     g->E = E;
     g->weighted = wt != nullptr;
     g->symmetric = cgpi::is_symmetric(h);
+    if (wt && E > 0) {
+      double sum = 0;
+      for (int64_t e = 0; e < E; ++e) sum += wt[e];
+      g->mean_weight = sum / (double)E;
+    }
     HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
     HIP_CHECK(hipMalloc((void**)&g->idx, (size_t)E * 4 + 4));
     HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
@@ -1730,7 +1738,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->dense_threshold = 8;
   o->frontier_dense_div = 8;
   o->dense_unroll = 8;
-  o->sssp_delta = 0;
+  o->sssp_delta = -1;
   o->graph_iterations = 8;
   o->profile = 0;
   o->hbm_budget_bytes = 64ull << 30;
@@ -1768,7 +1776,17 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
     if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8 && en->opts.dense_unroll != 16)
       cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4, 8 or 16");
-    if (en->opts.sssp_delta < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= 0");
+    if (en->opts.sssp_delta < -1) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= -1");
+    if (en->opts.sssp_delta == -1) {
+      // Automatic near-far step.  Low-degree weighted graphs (grids, road
+      // networks: long shortest paths, many re-relaxations) get twenty mean
+      // weights (on a 4096^2 grid that is faster than no step both for one
+      // query and for 512); small-world graphs settle in few Bellman-Ford
+      // iterations and skip it.
+      const double deg = g->V > 0 ? (double)g->E / (double)g->V : 0.0;
+      en->opts.sssp_delta =
+          (g->weighted && deg <= 8.0 && g->mean_weight > 1.0) ? (int32_t)std::lround(20.0 * g->mean_weight) : 0;
+    }
     en->cus = device_cus();
     {
       int dev = 0;

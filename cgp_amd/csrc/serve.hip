@@ -24,21 +24,30 @@ using namespace cgpi;
 
 namespace {
 
+// Up to SNAP_MAX converged slots per launch, passed by value (no host
+// staging buffer to keep alive): entry = word << 24 | lane << 16 | row.
+constexpr int SNAP_MAX = 64;
+struct SnapArgs {
+  int n;
+  u32 ent[SNAP_MAX];
+  u64 clear[MAXW];  // visited bits to clear, per word
+};
+
 // For each listed slot (word, lane) -> output row: copies the slot's value
-// column into out[row * V + v], then resets the column to INF and clears the
-// slot's visited bit, leaving the slot ready for its next query.
+// column into out[row * V + v] (when out is set), then resets the column to
+// INF and clears the slot's visited bit, leaving the slot ready for its next
+// query.
 __global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visited, int32_t V, int W,
-                                                          int rshift, const int32_t* slot_word,
-                                                          const int32_t* slot_lane, const int32_t* slot_row,
-                                                          int nslots, const u64* clear_mask, u32* out) {
+                                                          int rshift, SnapArgs a, u32* out) {
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
-    for (int k = 0; k < nslots; ++k) {
-      const size_t at = (((size_t)slot_word[k] * (size_t)V + (size_t)v) << rshift) + (size_t)slot_lane[k];
-      if (out) out[(size_t)slot_row[k] * V + v] = vals[at];
+    for (int k = 0; k < a.n; ++k) {
+      const u32 e = a.ent[k];
+      const size_t at = (((size_t)(e >> 24) * (size_t)V + (size_t)v) << rshift) + ((e >> 16) & 0xFFu);
+      if (out) out[(size_t)(e & 0xFFFFu) * V + v] = vals[at];
       vals[at] = INF;
     }
     for (int w = 0; w < W; ++w)
-      if (clear_mask[w]) visited[(size_t)v * (2 * W) + w] &= ~clear_mask[w];  // visited half of the record
+      if (a.clear[w]) visited[(size_t)v * (2 * W) + w] &= ~a.clear[w];  // visited half of the record
   }
 }
 
@@ -73,8 +82,23 @@ struct ServeState {
   // WCC labels of the session (computed on the first WCC request).
   bool wcc_ready = false;
   std::vector<u32> wcc_host;
-  // Device scratch for snapshots.
-  cgpi::DevBuf d_word, d_lane, d_row, d_mask, d_rows_out, d_src;
+  cgpi::DevBuf d_src;  // seed sources
+  // The pinned staging that admissions upload from (slot configuration and
+  // seed sources) is rewritten only after the previous upload has run.
+  hipEvent_t staged = nullptr;
+  bool staged_pending = false;
+  // Results copy to host on their own stream while the traversal goes on:
+  // a retired group's snapshot rows sit in a device staging buffer until
+  // its copy event completes, then the completions are published.
+  hipStream_t cs = nullptr;
+  struct Copying {
+    hipEvent_t snap = nullptr, copied = nullptr;
+    std::vector<cgp_completion> comps;
+    cgpi::DevBuf* staging = nullptr;
+  };
+  std::deque<Copying> copying;
+  int copying_n = 0;  // completions in flight
+  std::vector<cgpi::DevBuf*> staging_free;
   double clock() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -111,6 +135,11 @@ void hand_out(cgp_engine* en, uint32_t* b) {
 // Admits queued queries into free slots (within the budget) and seeds them.
 void admit(cgp_engine* en, ServeState* ss) {
   const int32_t V = en->g->V;
+  if (ss->queue.empty()) return;
+  if (ss->staged_pending) {
+    HIP_CHECK(hipEventSynchronize(ss->staged));
+    ss->staged_pending = false;
+  }
   Cfg& cfg = *en->h_cfg;
   int32_t* h_src = en->h_stage;
   bool any = false;
@@ -143,12 +172,23 @@ void admit(cgp_engine* en, ServeState* ss) {
       continue;
     }
     if (ss->busy >= ss->budget_slots) break;
+    // SSSP queries take the lowest free slot and BFS queries the highest, so
+    // SSSP slots stay packed into the fewest words (their value rows are
+    // what the advance reads) and BFS slots share the others.
     int slot = -1;
-    for (int k = 0; k < ss->nslots; ++k)
-      if (!ss->slots[k].busy) {
-        slot = k;
-        break;
-      }
+    if (pd.q.kind == CGP_SSSP) {
+      for (int k = 0; k < ss->nslots; ++k)
+        if (!ss->slots[k].busy) {
+          slot = k;
+          break;
+        }
+    } else {
+      for (int k = ss->nslots - 1; k >= 0; --k)
+        if (!ss->slots[k].busy) {
+          slot = k;
+          break;
+        }
+    }
     if (slot < 0) break;
     ss->queue.pop_front();
     SlotState& st = ss->slots[slot];
@@ -172,6 +212,8 @@ void admit(cgp_engine* en, ServeState* ss) {
   if (!any) return;
   HIP_CHECK(hipMemcpyAsync(en->d_cfg, en->h_cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipMemcpyAsync(ss->d_src.p, h_src, (size_t)ss->nslots * 4, hipMemcpyHostToDevice, en->s));
+  HIP_CHECK(hipEventRecord(ss->staged, en->s));
+  ss->staged_pending = true;
   if (ss->need_restart) {
     HIP_CHECK(hipMemsetAsync(&en->d_ctl->finished, 0, 4, en->s));
     ss->need_restart = false;
@@ -179,46 +221,69 @@ void admit(cgp_engine* en, ServeState* ss) {
   dispatch_seed_only(en, ss->W, en->kp(), (const int32_t*)ss->d_src.p, ss->nslots);
 }
 
-// Snapshots converged slots, recycles them and queues their completions.
+cgpi::DevBuf* take_staging(ServeState* ss, size_t bytes) {
+  for (size_t i = 0; i < ss->staging_free.size(); ++i) {
+    cgpi::DevBuf* b = ss->staging_free[i];
+    if (b->bytes >= bytes) {
+      ss->staging_free.erase(ss->staging_free.begin() + (long)i);
+      return b;
+    }
+  }
+  cgpi::DevBuf* b = new cgpi::DevBuf();
+  b->ensure(bytes);
+  return b;
+}
+
+// Publishes the completions of finished result copies (oldest first); with
+// `wait`, waits for all of them.
+void poll_copies(ServeState* ss, bool wait) {
+  while (!ss->copying.empty()) {
+    ServeState::Copying& c = ss->copying.front();
+    if (wait) {
+      HIP_CHECK(hipEventSynchronize(c.copied));
+    } else {
+      const hipError_t q = hipEventQuery(c.copied);
+      if (q == hipErrorNotReady) break;
+      HIP_CHECK(q);
+    }
+    const double now = ss->clock();
+    for (cgp_completion& x : c.comps) {
+      x.done_s = now;
+      ss->done.push_back(x);
+    }
+    ss->copying_n -= (int)c.comps.size();
+    (void)hipEventDestroy(c.snap);
+    (void)hipEventDestroy(c.copied);
+    ss->staging_free.push_back(c.staging);
+    ss->copying.pop_front();
+  }
+}
+
+// Snapshots converged slots and recycles them at once (the reset is ordered
+// before any later seed on the engine stream); their results reach host
+// memory on the copy stream and complete in poll_copies.
 void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
   const int32_t V = en->g->V;
   const int k = (int)conv.size();
-  int32_t* h = en->h_stage;  // pinned: [word k][lane k][row k][pad][mask W]
-  u64* mask = reinterpret_cast<u64*>(h + ((3 * k + 1) & ~1));
-  for (int w = 0; w < ss->W; ++w) mask[w] = 0ull;
-  for (int i = 0; i < k; ++i) {
-    h[i] = conv[i] >> 6;
-    h[k + i] = conv[i] & 63;
-    h[2 * k + i] = i;
-    mask[conv[i] >> 6] |= 1ull << (conv[i] & 63);
-  }
-  ss->d_word.ensure((size_t)k * 4);
-  ss->d_lane.ensure((size_t)k * 4);
-  ss->d_row.ensure((size_t)k * 4);
-  ss->d_mask.ensure((size_t)ss->W * 8);
-  if (ss->keep) ss->d_rows_out.ensure((size_t)k * V * 4);
-  HIP_CHECK(hipMemcpyAsync(ss->d_word.p, h, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
-  HIP_CHECK(hipMemcpyAsync(ss->d_lane.p, h + k, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
-  HIP_CHECK(hipMemcpyAsync(ss->d_row.p, h + 2 * k, (size_t)k * 4, hipMemcpyHostToDevice, en->s));
-  HIP_CHECK(hipMemcpyAsync(ss->d_mask.p, mask, (size_t)ss->W * 8, hipMemcpyHostToDevice, en->s));
-  hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
-                     (u64*)en->visited.p, V, ss->W, 6, (const int32_t*)ss->d_word.p,
-                     (const int32_t*)ss->d_lane.p, (const int32_t*)ss->d_row.p, k,
-                     (const u64*)ss->d_mask.p, ss->keep ? (u32*)ss->d_rows_out.p : nullptr);
-  HIP_CHECK(hipGetLastError());
-  std::vector<uint32_t*> bufs((size_t)k, nullptr);
-  if (ss->keep) {
-    for (int i = 0; i < k; ++i) {
-      bufs[i] = take_buffer(en, (size_t)V);
-      HIP_CHECK(hipMemcpyAsync(bufs[i], (const u32*)ss->d_rows_out.p + (size_t)i * V, (size_t)V * 4,
-                               hipMemcpyDeviceToHost, en->s));
+  cgpi::DevBuf* staging = ss->keep ? take_staging(ss, (size_t)k * V * 4) : nullptr;
+  for (int c0 = 0; c0 < k; c0 += SNAP_MAX) {
+    SnapArgs a{};
+    a.n = std::min(SNAP_MAX, k - c0);
+    for (int i = 0; i < a.n; ++i) {
+      const int slot = conv[c0 + i];
+      a.ent[i] = ((u32)(slot >> 6) << 24) | ((u32)(slot & 63) << 16) | (u32)(c0 + i);
+      a.clear[slot >> 6] |= 1ull << (slot & 63);
     }
+    hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
+                       (u64*)en->visited.p, V, ss->W, 6, a, staging ? (u32*)staging->p : nullptr);
+    HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipStreamSynchronize(en->s));
+  std::vector<cgp_completion> comps((size_t)k);
   const double now = ss->clock();
   for (int i = 0; i < k; ++i) {
     SlotState& st = ss->slots[conv[i]];
-    cgp_completion c{};
+    cgp_completion& c = comps[i];
+    c = cgp_completion{};
     c.ticket = st.ticket;
     c.kind = st.q.kind;
     c.source = st.q.source;
@@ -226,25 +291,47 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
     c.admit_s = st.admit;
     c.done_s = now;
     c.iterations = (int32_t)(ss->iter - st.start + 1);
-    c.result = bufs[i];
-    if (bufs[i]) hand_out(en, bufs[i]);
-    ss->done.push_back(c);
     st.busy = false;
     --ss->busy;
   }
+  if (!ss->keep) {
+    for (const cgp_completion& c : comps) ss->done.push_back(c);
+    return;
+  }
+  ServeState::Copying g;
+  HIP_CHECK(hipEventCreateWithFlags(&g.snap, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&g.copied, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(g.snap, en->s));
+  HIP_CHECK(hipStreamWaitEvent(ss->cs, g.snap, 0));
+  for (int i = 0; i < k; ++i) {
+    comps[i].result = take_buffer(en, (size_t)V);
+    hand_out(en, comps[i].result);
+    HIP_CHECK(hipMemcpyAsync(comps[i].result, (const u32*)staging->p + (size_t)i * V, (size_t)V * 4,
+                             hipMemcpyDeviceToHost, ss->cs));
+  }
+  HIP_CHECK(hipEventRecord(g.copied, ss->cs));
+  g.comps = std::move(comps);
+  g.staging = staging;
+  ss->copying_n += k;
+  ss->copying.push_back(std::move(g));
 }
 
 void serve_free(cgp_engine* en) {
   ServeState* ss = en->serve;
   if (!ss) return;
   (void)hipStreamSynchronize(en->s);
+  try {
+    poll_copies(ss, true);
+  } catch (...) {
+  }
+  if (ss->cs) (void)hipStreamDestroy(ss->cs);
+  if (ss->staged) (void)hipEventDestroy(ss->staged);
   for (auto& c : ss->done)
     if (c.result) cgp_serve_release_result(c.result);
-  ss->d_word.release();
-  ss->d_lane.release();
-  ss->d_row.release();
-  ss->d_mask.release();
-  ss->d_rows_out.release();
+  for (cgpi::DevBuf* b : ss->staging_free) {
+    b->release();
+    delete b;
+  }
   ss->d_src.release();
   // Stamps used by the session must not recur in later runs; slots may hold
   // bits in next[] when stopped mid-run.
@@ -325,6 +412,8 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     HIP_CHECK(hipMemcpyAsync(en->d_cfg, &cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
+    HIP_CHECK(hipStreamCreateWithFlags(&ss->cs, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&ss->staged, hipEventDisableTiming));
     ss->t0 = std::chrono::steady_clock::now();
     en->serve = ss.release();
   });
@@ -354,7 +443,11 @@ cgp_status cgp_serve_step(cgp_engine* en, int32_t max_iterations, cgp_completion
     std::vector<int> conv;
     for (;;) {
       admit(en, ss);
-      if (ss->busy == 0) break;  // idle: nothing admitted or running
+      poll_copies(ss, false);
+      if (ss->busy == 0) {  // no traversal left: finish the result copies
+        poll_copies(ss, true);
+        break;
+      }
       if (max_iterations > 0 && ran >= max_iterations) break;
       dispatch_iteration(en, W, en->kp());
       HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
@@ -372,6 +465,7 @@ cgp_status cgp_serve_step(cgp_engine* en, int32_t max_iterations, cgp_completion
       }
       if (hc.finished) ss->need_restart = true;
       if (!conv.empty()) retire(en, ss, conv);
+      poll_copies(ss, false);
       if (max_iterations == 0 && !ss->done.empty()) break;
       // Every query converges within V + 1 iterations of its admission per
       // raise of the near-far threshold.

@@ -92,7 +92,9 @@ typedef struct {
   int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
                                   wait; the threshold moves to the smallest waiting value + delta
                                   when nothing below it is left.  0 = no waiting (Bellman-Ford
-                                  frontier).  Results are identical either way. */
+                                  frontier); -1 (default) = automatic: 20 x mean edge weight on
+                                  weighted graphs of average degree <= 8, else 0.  Results are
+                                  identical either way. */
 } cgp_engine_opts;
 
 /* Per-run report; replaces run_result_t / iteration_profile_t
@@ -255,6 +257,9 @@ cgp_status cgp_serve_step(cgp_engine* e, int32_t max_iterations, cgp_completion*
                           int32_t cap, int32_t* n_done);
 /* Engine clock (seconds since cgp_serve_start), same clock as admit_s / done_s. */
 double cgp_serve_clock(const cgp_engine* e);
+/* queued: waiting for a slot; running: holding a slot.  A converged query frees its
+ * slot at once and completes when its result reaches host memory (the copy runs while
+ * the traversal continues); cgp_serve_step on an idle engine finishes those copies. */
 cgp_status cgp_serve_pending(const cgp_engine* e, int32_t* queued, int32_t* running);
 void cgp_serve_release_result(uint32_t* result);
 cgp_status cgp_serve_stop(cgp_engine* e);

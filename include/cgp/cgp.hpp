@@ -161,7 +161,7 @@ struct run_options {
   int dense_threshold = 0;
   int frontier_dense_div = 0;
   int dense_unroll = 0;
-  int sssp_delta = -1;  // -1: engine default
+  int sssp_delta = -1;  // -1: automatic (engine default)
 
   cgp_engine_opts to_c() const {
     cgp_engine_opts o;
@@ -181,7 +181,7 @@ struct run_options {
     if (dense_threshold) o.dense_threshold = dense_threshold;
     if (frontier_dense_div) o.frontier_dense_div = frontier_dense_div;
     if (dense_unroll) o.dense_unroll = dense_unroll;
-    if (sssp_delta >= 0) o.sssp_delta = sssp_delta;
+    o.sssp_delta = sssp_delta;
     return o;
   }
 };

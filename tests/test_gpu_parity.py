@@ -163,10 +163,21 @@ def test_near_far_grid_long_paths(gpu):
     g = cgp_amd.Graph.from_host(h)
     og = O.G.of(h)
     src = [0, 4095, 2080, 17]
-    ref = cgp_amd.Engine(g).run_batch([SSSP] * 4, src)[0]
+    ref = cgp_amd.Engine(g, sssp_delta=0).run_batch([SSSP] * 4, src)[0]
     out, st = cgp_amd.Engine(g, sssp_delta=50).run_batch([SSSP] * 4, src)
     assert np.array_equal(out, ref)
     assert np.array_equal(out, oracle_rows(og, [1] * 4, src))
+    # Automatic step: low-degree weighted graph -> 20 x mean weight, and the
+    # same results.
+    en = cgp_amd.Engine(g)
+    w = np.asarray(h.weights, dtype=np.float64)
+    assert en.opts["sssp_delta"] == int(round(20 * w.mean()))
+    assert np.array_equal(en.run_batch([SSSP] * 4, src)[0], ref)
+
+
+def test_near_far_automatic_off_on_small_world(gpu):
+    h, g, og = rmat(12, 16, 4, 1, 255)
+    assert cgp_amd.Engine(g).opts["sssp_delta"] == 0
 
 
 def test_alone_equals_in_the_mix(gpu):
