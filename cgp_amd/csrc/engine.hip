@@ -1225,7 +1225,7 @@ namespace {
 
 // Both frontier-mode instantiations of the lane-per-query advance (the one
 // not matching the device-selected mode exits at once), U = rows per step.
-// It is wave-level work, so it always runs 256-thread workgroups, 8 per CU.
+// It is wave-level work, so it always runs 256-thread workgroups.
 template <int W, int U>
 void launch_dense_u(hipStream_t s, dim3 grid, const KP& P) {
   hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U>), grid, dim3(BLOCK), 0, s, P);
@@ -1233,8 +1233,8 @@ void launch_dense_u(hipStream_t s, dim3 grid, const KP& P) {
 }
 
 template <int W>
-void launch_dense(hipStream_t s, int cus, const KP& P, int unroll) {
-  const dim3 grid((unsigned)cus * 8u);
+void launch_dense(hipStream_t s, int blocks, const KP& P, int unroll) {
+  const dim3 grid((unsigned)blocks);
   if (unroll == 4) launch_dense_u<W, 4>(s, grid, P);
   else if (unroll == 16) launch_dense_u<W, 16>(s, grid, P);
   else launch_dense_u<W, 8>(s, grid, P);
@@ -1246,7 +1246,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
   const int bs = en->opts.block_size;
   const dim3 grid(en->grid_advance());
-  launch_dense<W>(s, en->cus, P, en->opts.dense_unroll);
+  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll);
   if (bs == 128) {
     hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
   } else if (bs == 512) {
@@ -1479,11 +1479,11 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     }
     // Bottom-up steps pay off only when few BFS queries share the words: with
     // many, the missing bits of a vertex are rarely all found early.  Hybrid
-    // mode therefore pulls only for batches of at most kPullMaxQueries BFS
-    // queries; CGP_PULL forces it.
-    constexpr int kPullMaxQueries = 4;
+    // mode pulls only for batches of at most pull_max_queries BFS queries
+    // (default 0: on MI355X the push kernels measured faster even for one
+    // BFS on RMAT-22); CGP_PULL forces it.
     const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH &&
-                          (en->opts.traversal_mode == CGP_PULL || nb <= kPullMaxQueries);
+                          (en->opts.traversal_mode == CGP_PULL || nb <= en->opts.pull_max_queries);
     cfg.pull_policy = !can_pull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
     cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
     cfg.pull_pairs_thr = (u64)(en->opts.pull_frontier_ratio * (double)nb * (double)V);
@@ -1774,6 +1774,8 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.dense_threshold <= 0) en->opts.dense_threshold = d.dense_threshold;
     if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
     if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
+    if (en->opts.advance_blocks < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: advance_blocks must be >= 0");
+    if (en->opts.pull_max_queries < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: pull_max_queries must be >= 0");
     if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8 && en->opts.dense_unroll != 16)
       cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4, 8 or 16");
     if (en->opts.sssp_delta < -1) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= -1");

@@ -182,7 +182,15 @@ struct cgp_engine {
   // Grid sizes: persistent-style, sized to the device, independent of the
   // frontier so the launches can be captured once and replayed.
   // Same thread count per launch for every workgroup size.
-  int grid_advance() const { return cus * 8 * 256 / std::max(128, opts.block_size); }
+  // Advance workgroups: `advance_blocks`, or by default one per 8192 edges
+  // between one and eight per CU (small graphs would otherwise spend each
+  // iteration on thousands of waves contending for empty work cursors).
+  int advance_blocks() const {
+    if (opts.advance_blocks > 0) return opts.advance_blocks;
+    const int64_t want = g->E / 8192;
+    return (int)std::max<int64_t>(cus, std::min<int64_t>(want, (int64_t)cus * 8));
+  }
+  int grid_advance() const { return (int)((int64_t)advance_blocks() * 256 / std::max(128, opts.block_size)); }
   int grid_filter() const {
     int64_t need = ((int64_t)g->V + BLOCK - 1) / BLOCK;
     return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 8));

@@ -89,6 +89,10 @@ typedef struct {
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
   int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4, 8
                                   or 16 (fewer registers, more waves vs more rows per wave); 0 = 8 */
+  int32_t pull_max_queries;    /* hybrid mode takes bottom-up steps only for batches of at most
+                                  this many BFS queries (0 = never; CGP_PULL always pulls) */
+  int32_t advance_blocks;      /* workgroups of the advance launches (256-thread equivalents);
+                                  0 = one per 8192 edges, between 1 and 8 per CU */
   int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
                                   wait; the threshold moves to the smallest waiting value + delta
                                   when nothing below it is left.  0 = no waiting (Bellman-Ford

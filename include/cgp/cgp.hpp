@@ -161,6 +161,8 @@ struct run_options {
   int dense_threshold = 0;
   int frontier_dense_div = 0;
   int dense_unroll = 0;
+  int pull_max_queries = 0;
+  int advance_blocks = 0;
   int sssp_delta = -1;  // -1: automatic (engine default)
 
   cgp_engine_opts to_c() const {
@@ -181,6 +183,8 @@ struct run_options {
     if (dense_threshold) o.dense_threshold = dense_threshold;
     if (frontier_dense_div) o.frontier_dense_div = frontier_dense_div;
     if (dense_unroll) o.dense_unroll = dense_unroll;
+    o.pull_max_queries = pull_max_queries;
+    if (advance_blocks) o.advance_blocks = advance_blocks;
     o.sssp_delta = sssp_delta;
     return o;
   }

@@ -85,6 +85,19 @@ def test_rmat_bfs_batches(gpu, mode, nq):
     assert np.array_equal(out, oracle_rows(og, [0] * nq, src))
 
 
+@pytest.mark.parametrize("nq", [1, 3])
+def test_hybrid_pull_steps_for_small_batches(gpu, nq):
+    # pull_max_queries lets hybrid mode take bottom-up steps (the reference's
+    # frontier / edge-ratio thresholds); results stay bit-exact.
+    h, g, og = rmat(14, 16, 3)
+    src = h.sample_sources(nq, seed=11)
+    out, st = cgp_amd.Engine(g, pull_max_queries=4).run_batch([BFS] * nq, src)
+    assert st["pull_iterations"] > 0
+    assert np.array_equal(out, oracle_rows(og, [0] * nq, src))
+    out2, st2 = cgp_amd.Engine(g).run_batch([BFS] * nq, src)
+    assert st2["pull_iterations"] == 0 and np.array_equal(out2, out)
+
+
 @pytest.mark.parametrize("nq", [1, 5, 16, 64, 100])
 def test_rmat_sssp_batches(gpu, nq):
     say("rmat sssp nq", nq)
