@@ -76,6 +76,44 @@ __device__ __forceinline__ u32 wave_reserve(u32 cnt, u32* counter) {
   return base + x - cnt;
 }
 
+// Workgroup-level exclusive prefix sums of four per-thread counts, with one
+// atomicAdd per non-zero total reserving the block's range in counters[0..3];
+// at[k] is the thread's first slot for counter k.  Every thread of the
+// BLOCK-thread workgroup must call it (uniform control flow); lds holds
+// (BLOCK/64)*4 + 4 words.
+__device__ __forceinline__ void block_reserve4(const u32 (&cnt)[4], u32* counters, u32 (&at)[4], u32* lds) {
+  constexpr int NW = BLOCK / 64;
+  const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
+  u32 incl[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u32 x = cnt[k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const u32 y = (u32)__shfl_up((int)x, d, 64);
+      if (lane >= d) x += y;
+    }
+    incl[k] = x;
+    if (lane == 63) lds[wave * 4 + k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = (int)threadIdx.x;
+    u32 tot = 0;
+#pragma unroll
+    for (int wv = 0; wv < NW; ++wv) tot += lds[wv * 4 + k];
+    lds[NW * 4 + k] = tot ? atomicAdd(&counters[k], tot) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u32 off = lds[NW * 4 + k];
+    for (int wv = 0; wv < wave; ++wv) off += lds[wv * 4 + k];
+    at[k] = off + incl[k] - cnt[k];
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ u64 wave_sum64(u64 x) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) x += shfl64(x, lane_id() ^ d);
@@ -733,21 +771,29 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, dense_t, rshift,
                             stamp, thr, (u32)item == 0 && threadIdx.x == 0, t);
   }
+  // Work cursors are single addresses every wave contends for, so each
+  // fetch claims several units (FETCH_M medium vertices, FETCH_S groups of
+  // 64 small vertices).
+  constexpr u32 FETCH_M = 4, FETCH_S = 4;
   // Phase M.
   const u32 nM = C->bcount[p][1];
   while (nM) {
-    const u32 i = wave_fetch(&C->fetch[p][2], 1u);
-    if (i >= nM) break;
-    const int32_t u = P.medium[p][i];
-    const int32_t e0 = P.off[u], e1 = P.off[u + 1];
-    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
+    const u32 i0 = wave_fetch(&C->fetch[p][2], FETCH_M);
+    if (i0 >= nM) break;
+    const u32 i1 = min(nM, i0 + FETCH_M);
+    for (u32 i = i0; i < i1; ++i) {
+      const int32_t u = P.medium[p][i];
+      const int32_t e0 = P.off[u], e1 = P.off[u + 1];
+      expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
+    }
   }
   // Phase S.
   const u32 nS = C->bcount[p][0];
   while (nS) {
-    const u32 base = wave_fetch(&C->fetch[p][3], 64u);
-    if (base >= nS) break;
-    expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, thr, t);
+    const u32 b0 = wave_fetch(&C->fetch[p][3], 64u * FETCH_S);
+    if (b0 >= nS) break;
+    for (u32 base = b0; base < nS && base < b0 + 64u * FETCH_S; base += 64u)
+      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, thr, t);
   }
   // Phase P: high-degree vertices one wave each (64 neighbours per step,
   // early exit on a wave-wide OR), then the rest one lane each.
@@ -826,6 +872,8 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   const u32 thr = C->thr;
   const int rshift = P.cfg->rshift;
   Tally t;
+  // One group of 64 items per fetch: items carry up to DCHUNK edges each, so
+  // coarser claims leave a long tail (measured 2.5x slower at 4 groups).
   while (true) {
     const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
@@ -900,6 +948,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   Ctl* C = P.ctl;
   if (C->finished) return;
   __shared__ int s_last;
+  __shared__ u32 s_res[(BLOCK / 64) * 4 + 4];
   if (threadIdx.x == 0) atomicMin(&C->flt_t0, (u64)wall_clock64());
   const Cfg* cfg = P.cfg;
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
@@ -955,22 +1004,25 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       dmask = 0;
     }
     const int b = deg <= P.bin_small ? 0 : (deg <= P.bin_large ? 1 : 2);
-    wave_append<int32_t>(need_bin && b == 0, v, &C->bcount[np][0], P.small[np]);
-    wave_append<int32_t>(need_bin && b == 1, v, &C->bcount[np][1], P.medium[np]);
     {
-      const u32 lch = (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
-      const u32 at = wave_reserve(lch, &C->bcount[np][2]);
-      for (u32 c = 0; c < lch; ++c) P.large[np][at + c] = ((u64)(u32)v << 32) | c;
-    }
-    {
+      // One reservation per workgroup and bin (4 global atomics per 256
+      // vertices instead of 4 per wave: the bin counters are single
+      // addresses every workgroup contends for).
       const u32 dch = (u32)((deg + DCHUNK - 1) / DCHUNK);
-      const u32 cnt = (u32)__popcll(dmask) * dch;
-      const u32 at = wave_reserve(cnt, &C->bcount[np][3]);
+      u32 cnt[4], at[4];
+      cnt[0] = (need_bin && b == 0) ? 1u : 0u;
+      cnt[1] = (need_bin && b == 1) ? 1u : 0u;
+      cnt[2] = (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
+      cnt[3] = (u32)__popcll(dmask) * dch;
+      block_reserve4(cnt, C->bcount[np], at, s_res);
+      if (cnt[0]) P.small[np][at[0]] = v;
+      if (cnt[1]) P.medium[np][at[1]] = v;
+      for (u32 c = 0; c < cnt[2]; ++c) P.large[np][at[2] + c] = ((u64)(u32)v << 32) | c;
       u32 k = 0;
       for (int w = 0; w < W; ++w) {
         if (!((dmask >> w) & 1u)) continue;
         for (u32 c = 0; c < dch; ++c)
-          P.dense[np][at + k++] = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
+          P.dense[np][at[3] + k++] = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
       }
     }
     // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
