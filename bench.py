@@ -99,14 +99,18 @@ def cpu_baseline(h, kinds, sources, seconds_target):
     }
 
 
-def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R):
-    """HBM bytes per working dispatch of `kernel_prefix` from rocprofv3
-    FETCH_SIZE / WRITE_SIZE CSVs (KiB per dispatch), each scaled by the factor
-    that makes the same counter read the known bytes of k_output (reads the
-    words*V*R value rows, writes out_rows*V*4 bytes) in the same run."""
+def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R, runs, launches_per_run):
+    """HBM bytes per working launch of `kernel_prefix` from rocprofv3
+    FETCH_SIZE / WRITE_SIZE CSVs (KiB per dispatch): each counter's total over
+    the profiled run, scaled by the factor that makes the same counter read
+    the known bytes of k_output (reads the words*V*R value rows, writes
+    out_rows*V*4 bytes), divided by the working launches of that run (`runs`
+    batch runs of `launches_per_run` launches that did work each, the same
+    launches the algorithmic bytes are averaged over)."""
     import csv
     import glob
     tot = 0.0
+    factors = {}
     for counter, known in (("FETCH_SIZE", 4.0 * words * V * R), ("WRITE_SIZE", 4.0 * out_rows * V)):
         vals, cal = [], []
         for f in glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
@@ -122,14 +126,9 @@ def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R):
         if not vals or not cal:
             return None, None
         factor = known / (sum(cal) / len(cal))
-        top = max(vals)
-        work = [v for v in vals if v > 0.01 * top]
-        tot += factor * sum(work) / len(work)
-        if counter == "FETCH_SIZE":
-            fetch_factor = factor
-        else:
-            write_factor = factor
-    return tot, {"fetch_factor": fetch_factor, "write_factor": write_factor}
+        tot += factor * sum(vals)
+        factors[counter.lower() + "_factor"] = factor
+    return tot / max(1.0, runs * launches_per_run), factors
 
 
 def make_graph(spec, seed, wmin, wmax):
@@ -254,6 +253,8 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--pmc-dir", default="",
                     help="directory with rocprofv3 FETCH_SIZE/WRITE_SIZE CSVs of this workload: fills roofline.traffic")
+    ap.add_argument("--pmc-runs", type=int, default=3,
+                    help="batch runs in each profiled command (bench.py --steps 1 --warmup 0 makes 3)")
     ap.add_argument("--no-one-at-a-time", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -386,7 +387,8 @@ def main():
     if args.pmc_dir:
         words = (n_bfs + n_sssp + 63) // 64
         R = 64 if (n_bfs + n_sssp) > 64 else 1 << max(0, (n_bfs + n_sssp - 1).bit_length())
-        traffic, factors = pmc_traffic(args.pmc_dir, dom + "<", n_bfs + n_sssp, V, words, R)
+        traffic, factors = pmc_traffic(args.pmc_dir, dom + "<", n_bfs + n_sssp, V, words, R, args.pmc_runs,
+                                       k_launch / max(1, args.steps))
         if traffic:
             result["roofline"]["traffic"] = traffic
             result["roofline"]["traffic_calibration"] = factors
