@@ -46,6 +46,7 @@ class cgp_engine_opts(C.Structure):
         ("dense_threshold", C.c_int32),
         ("frontier_dense_div", C.c_int32),
         ("dense_unroll", C.c_int32),
+        ("dense_narrow", C.c_int32),
         ("pull_max_queries", C.c_int32),
         ("advance_blocks", C.c_int32),
         ("sssp_delta", C.c_int32),

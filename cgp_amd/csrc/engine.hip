@@ -16,12 +16,14 @@
 // each component on symmetric graphs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "engine_impl.hpp"
@@ -76,17 +78,18 @@ __device__ __forceinline__ u32 wave_reserve(u32 cnt, u32* counter) {
   return base + x - cnt;
 }
 
-// Workgroup-level exclusive prefix sums of four per-thread counts, with one
-// atomicAdd per non-zero total reserving the block's range in counters[0..3];
+// Workgroup-level exclusive prefix sums of N per-thread counts, with one
+// atomicAdd per non-zero total reserving the block's range in counters[0..N);
 // at[k] is the thread's first slot for counter k.  Every thread of the
 // BLOCK-thread workgroup must call it (uniform control flow); lds holds
-// (BLOCK/64)*4 + 4 words.
-__device__ __forceinline__ void block_reserve4(const u32 (&cnt)[4], u32* counters, u32 (&at)[4], u32* lds) {
+// (BLOCK/64 + 1) * N words.
+template <int N>
+__device__ __forceinline__ void block_reserve(const u32 (&cnt)[N], u32* counters, u32 (&at)[N], u32* lds) {
   constexpr int NW = BLOCK / 64;
   const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
-  u32 incl[4];
+  u32 incl[N];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < N; ++k) {
     u32 x = cnt[k];
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -94,21 +97,21 @@ __device__ __forceinline__ void block_reserve4(const u32 (&cnt)[4], u32* counter
       if (lane >= d) x += y;
     }
     incl[k] = x;
-    if (lane == 63) lds[wave * 4 + k] = x;
+    if (lane == 63) lds[wave * N + k] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < N) {
     const int k = (int)threadIdx.x;
     u32 tot = 0;
 #pragma unroll
-    for (int wv = 0; wv < NW; ++wv) tot += lds[wv * 4 + k];
-    lds[NW * 4 + k] = tot ? atomicAdd(&counters[k], tot) : 0u;
+    for (int wv = 0; wv < NW; ++wv) tot += lds[wv * N + k];
+    lds[NW * N + k] = tot ? atomicAdd(&counters[k], tot) : 0u;
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    u32 off = lds[NW * 4 + k];
-    for (int wv = 0; wv < wave; ++wv) off += lds[wv * 4 + k];
+  for (int k = 0; k < N; ++k) {
+    u32 off = lds[NW * N + k];
+    for (int wv = 0; wv < wave; ++wv) off += lds[wv * N + k];
     at[k] = off + incl[k] - cnt[k];
   }
   __syncthreads();
@@ -630,6 +633,125 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   }
 }
 
+// Lane K of each 16-lane row, broadcast to the whole row (DPP row_newbcast).
+template <int K>
+__device__ __forceinline__ u32 row_bcast(u32 x) {
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x150 + K, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ u64 row_bcast64(u64 x) {
+  return ((u64)row_bcast<K>((u32)(x >> 32)) << 32) | row_bcast<K>((u32)x);
+}
+
+// Phase N: dense SSSP items of narrow words (every SSSP slot of the word in
+// lanes 0..15, rows of at least 16 lanes).  The 64 flattened edges of a
+// window are split by 16-lane row: row g relaxes edges 16g..16g+15, one per
+// step, its lanes being the word's sixteen queries, so a wave step relaxes
+// four edges.  A step's row pointers, weight and mask reach the row by a
+// DPP row broadcast from the lane that flattened the edge.  Otherwise as
+// phase D (no near-far split: the filter routes items here only when
+// postponing is off).
+template <int W, bool SPARSE>
+__device__ __forceinline__ void expand_flat_narrow(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
+                                                   Tally& t) {
+  const int lane = lane_id();
+  const u32 p = P.ctl->iter & 1u;
+  const bool have = base + (u32)lane < n;
+  const u64 item = have ? P.dense_n[p][base + lane] : 0ull;
+  const int32_t u = (int32_t)(item >> 32);
+  const int w = (int)((item >> 24) & 0xFF);
+  const u32 chunk = (u32)(item & 0xFFFFFF);
+  const int32_t e0 = have ? P.off[u] + (int32_t)(chunk * DCHUNK) : 0;
+  const int32_t e1 = have ? min(P.off[u + 1], e0 + DCHUNK) : 0;
+  const u32 sm = have ? (u32)(P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) & 0xFFFFu : 0u;
+  const u32 d = sm ? (u32)(e1 - e0) : 0u;
+  if (have) {
+    ++t.ditems;
+    t.dedges += d;
+    t.sssp += (u64)d * (u64)__popc(sm);
+  }
+  const u32 incl = wave_incl_scan(d);
+  const u32 total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
+  const u32 item_row = (u32)w * (u32)P.V + (u32)u;
+  const u64 vals_base = (u64)P.vals;
+  const int rsh = rshift + 2;
+  const int g = lane >> 4;
+  const u32 ql = (u32)(lane & 15), ql4 = ql * 4u;
+  for (u32 win = 0; win < total; win += 64) {
+    const u32 f = win + (u32)lane;
+    const int j = wave_upper_bound(incl, f);
+    const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
+    const int32_t ej = __shfl(e0, j, 64) + (int32_t)(f - exj);
+    const bool ok = f < total;
+    const int32_t myv = ok ? P.idx[ej] : 0;
+    const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : 0u;
+    const int myword = __shfl(w, j, 64);
+    const u64 my_vaddr = vals_base + ((u64)((u32)myword * (u32)P.V + (u32)myv) << rsh);
+    const u64 my_uaddr = vals_base + ((u64)(u32)__shfl((int)item_row, j, 64) << rsh);
+    const u32 am_j = (u32)__shfl((int)sm, j, 64);  // every lane active for the shuffle
+    const u32 my_am = ok ? am_j : 0u;
+    const int cnt = (int)min(64u, total - win);
+    u32 mine = 0;
+    // Four steps per chunk: their eight row loads are issued together.
+    auto chunk = [&](auto k0c) {
+      constexpr int K0 = decltype(k0c)::value;
+      u64 va[4];
+      u32 du[4], cur[4], c[4];
+      bool act[4];
+      auto load = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int i = K - K0;
+        va[i] = row_bcast64<K>(my_vaddr);
+        const u64 ua = row_bcast64<K>(my_uaddr);
+        c[i] = row_bcast<K>(myw);
+        act[i] = ((row_bcast<K>(my_am) >> ql) & 1u) != 0;
+        du[i] = *(const gu32*)(ua + ql4);
+        cur[i] = *(const gu32*)(va[i] + ql4);
+      };
+      load(std::integral_constant<int, K0 + 0>{});
+      load(std::integral_constant<int, K0 + 1>{});
+      load(std::integral_constant<int, K0 + 2>{});
+      load(std::integral_constant<int, K0 + 3>{});
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const u32 cand = __builtin_elementwise_add_sat(du[i], c[i]);
+        const bool imp = act[i] && cand < cur[i];
+        if (imp) {
+          __hip_atomic_fetch_min((gu32*)(va[i] + ql4), cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          note_improved(t, cand, INF);
+          ++t.imp;
+        }
+        const u64 m = __ballot(imp);
+        if (m == 0) continue;
+        // Row g's bits are edge 16g+K's next-word bits; that edge's own lane
+        // (ql == K) publishes them.
+        const u32 mg = (u32)(m >> (16 * g)) & 0xFFFFu;
+        const bool own = ql == (u32)(K0 + i);
+        if (!SPARSE) {
+          mine = own ? mg : mine;
+        } else {
+          bool claimed = false;
+          if (own && mg) {
+            ++t.nnew;
+            if (atomicOr(&P.next[nxt_at<W>(myv, myword)], (u64)mg) == 0) claimed = claim_vertex(P.mark, myv, stamp);
+          }
+          wave_append<int32_t>(claimed, myv, &P.ctl->tcount, P.touched);
+        }
+      }
+    };
+    // Row 0 holds the window's first 16 edges, so a chunk starting at or
+    // past `cnt` has no edge in any row.
+    chunk(std::integral_constant<int, 0>{});
+    if (4 < cnt) chunk(std::integral_constant<int, 4>{});
+    if (8 < cnt) chunk(std::integral_constant<int, 8>{});
+    if (12 < cnt) chunk(std::integral_constant<int, 12>{});
+    if (!SPARSE && mine) {
+      ++t.nnew;
+      atomicOr(&P.next[nxt_at<W>(myv, myword)], (u64)mine);
+    }
+  }
+}
+
 // Bottom-up BFS step for vertex v (one lane) on a symmetric graph: an
 // unvisited (v, q) becomes visited at depth L+1 iff some neighbour u has q
 // visited (all visited bits at this point have depth <= L, and a neighbour
@@ -895,6 +1017,45 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   }
 }
 
+// Phase N launch: the narrow-word lane-per-query items (both frontier modes
+// in one kernel; timed together with k_adv_dense).
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished) return;
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 nN = C->bcount[p][4];
+  if (nN == 0) return;
+  if (threadIdx.x == 0) {
+    const u64 now = (u64)wall_clock64();
+    atomicMin(&C->adv_t0, now);
+    atomicMin(&C->den_t0, now);
+  }
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift;
+  const bool sparse = C->dense_mode == 0;
+  Tally t;
+  while (true) {
+    const u32 i = wave_fetch(&C->fetch[p][5], 64u);
+    if (i >= nN) break;
+    if (sparse) expand_flat_narrow<W, true>(P, i, nN, rshift, stamp, t);
+    else expand_flat_narrow<W, false>(P, i, nN, rshift, stamp, t);
+  }
+  if (P.profile) {
+    prof_add(P, PF_DENSE_ITEMS, t.ditems);
+    prof_add(P, PF_DENSE_EDGES, t.dedges);
+    prof_add(P, PF_DENSE_PAIRS, t.sssp);
+    prof_add(P, PF_DENSE_IMP, t.imp);
+    prof_add(P, PF_DENSE_NEW, t.nnew);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u64 now = (u64)wall_clock64();
+    atomicMax(&C->adv_t1, now);
+    atomicMax(&C->den_t1, now);
+  }
+}
+
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void k_adv_push(KP P) {
   Ctl* C = P.ctl;
@@ -948,7 +1109,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   Ctl* C = P.ctl;
   if (C->finished) return;
   __shared__ int s_last;
-  __shared__ u32 s_res[(BLOCK / 64) * 4 + 4];
+  __shared__ u32 s_res[(BLOCK / 64 + 1) * 5];
   if (threadIdx.x == 0) atomicMin(&C->flt_t0, (u64)wall_clock64());
   const Cfg* cfg = P.cfg;
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
@@ -979,7 +1140,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     const int32_t deg = anyw ? P.off[v + 1] - P.off[v] : 0;
     u64 nb[W];
     bool need_bin = false;
-    u32 dmask = 0, vpairs = 0;
+    u32 dmask = 0, nmask = 0, vpairs = 0;  // dense words (wide / narrow)
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const u64 nw = nwv[w];
@@ -990,7 +1151,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       nb[w] = nw & bb[w];
       if (nb[w]) P.visited[vis_at<W>(v, w)] |= nb[w];
       const u64 s = nw & vb[w];
-      if (s && __popcll(s) >= dense_t) dmask |= 1u << w;
+      if (s && __popcll(s) >= dense_t) (((cfg->narrow_words >> w) & 1u) ? nmask : dmask) |= 1u << w;
       else if (s) need_bin = true;
       if (nb[w]) need_bin = true;
       un[w] |= nw;
@@ -1002,6 +1163,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     if (deg == 0) {
       need_bin = false;
       dmask = 0;
+      nmask = 0;
     }
     const int b = deg <= P.bin_small ? 0 : (deg <= P.bin_large ? 1 : 2);
     {
@@ -1009,20 +1171,24 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       // vertices instead of 4 per wave: the bin counters are single
       // addresses every workgroup contends for).
       const u32 dch = (u32)((deg + DCHUNK - 1) / DCHUNK);
-      u32 cnt[4], at[4];
+      u32 cnt[5], at[5];
       cnt[0] = (need_bin && b == 0) ? 1u : 0u;
       cnt[1] = (need_bin && b == 1) ? 1u : 0u;
       cnt[2] = (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
-      cnt[3] = (u32)__popcll(dmask) * dch;
-      block_reserve4(cnt, C->bcount[np], at, s_res);
+      cnt[3] = (u32)__popc(dmask) * dch;
+      cnt[4] = (u32)__popc(nmask) * dch;
+      block_reserve<5>(cnt, C->bcount[np], at, s_res);
       if (cnt[0]) P.small[np][at[0]] = v;
       if (cnt[1]) P.medium[np][at[1]] = v;
       for (u32 c = 0; c < cnt[2]; ++c) P.large[np][at[2] + c] = ((u64)(u32)v << 32) | c;
-      u32 k = 0;
+      u32 k = 0, kn = 0;
       for (int w = 0; w < W; ++w) {
-        if (!((dmask >> w) & 1u)) continue;
-        for (u32 c = 0; c < dch; ++c)
-          P.dense[np][at[3] + k++] = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
+        const bool wide = (dmask >> w) & 1u, narrow = (nmask >> w) & 1u;
+        for (u32 c = 0; c < dch && (wide || narrow); ++c) {
+          const u64 it = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
+          if (wide) P.dense[np][at[3] + k++] = it;
+          else P.dense_n[np][at[4] + kn++] = it;
+        }
       }
     }
     // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
@@ -1071,7 +1237,8 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   if (s_last && threadIdx.x == 0) {
     __threadfence();
     const u32 total = atomicAdd(&C->bcount[np][0], 0u) + atomicAdd(&C->bcount[np][1], 0u) +
-                      atomicAdd(&C->bcount[np][2], 0u) + atomicAdd(&C->bcount[np][3], 0u);
+                      atomicAdd(&C->bcount[np][2], 0u) + atomicAdd(&C->bcount[np][3], 0u) +
+                      atomicAdd(&C->bcount[np][4], 0u);
     const u64 fp = atomicAdd(&C->bfs_pairs[np], 0ull), fe = atomicAdd(&C->bfs_edges[np], 0ull);
     const u64 fall = atomicAdd(&C->fedges[np], 0ull);
     C->dense_mode = fall >= cfg->dense_fedges_thr ? 1u : 0u;
@@ -1079,10 +1246,10 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     for (int w = 0; w < MAXW; ++w) {
       C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
     }
-    const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3];
+    const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3] + C->bcount[p][4];
     C->tcount = 0;
-    C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = 0;
-    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = C->fetch[p][4] = 0;
+    C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = C->bcount[p][4] = 0;
+    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = C->fetch[p][4] = C->fetch[p][5] = 0;
     C->bfs_pairs[p] = 0;
     C->bfs_edges[p] = 0;
     C->done_blocks = 0;
@@ -1167,21 +1334,24 @@ __global__ void k_fill_u32(u32* p, size_t n, u32 val) {
 
 // Transposes the vertex-major value rows of word w into query-major output
 // rows: out[qmap[w*64+q] * V + v].  One workgroup per 64-vertex tile.
+// Transposes OUT_VT vertices x R slots of one word's value rows into the
+// query-major output (each output row segment is OUT_VT * 4 = 512 bytes).
+constexpr int OUT_VT = 128;
 __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, int32_t V, int rshift,
                                                   const int32_t* qmap, u32* out) {
-  __shared__ u32 tile[64][65];
+  __shared__ u32 tile[64][OUT_VT + 1];
   const int w = blockIdx.y;
   const int R = 1 << rshift;
-  const int32_t v0 = (int32_t)blockIdx.x * 64;
-  const int nv = min(64, V - v0);
+  const int32_t v0 = (int32_t)blockIdx.x * OUT_VT;
+  const int nv = min(OUT_VT, V - v0);
   const size_t rowbase = vrow(V, w, v0, rshift);
-  for (int t = threadIdx.x; t < 64 * R; t += BLOCK) {
+  for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
     const int j = t >> rshift, q = t & (R - 1);
     if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < 64 * R; t += BLOCK) {
-    const int q = t >> 6, j = t & 63;
+  for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
+    const int q = t / OUT_VT, j = t % OUT_VT;
     const int32_t dst = qmap[w * 64 + q];
     if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = tile[q][j];
   }
@@ -1223,20 +1393,61 @@ __global__ void k_wcc_init(int32_t* parent, int32_t V) {
     parent[v] = (int32_t)v;
 }
 
+// Unites the components of a and b: the larger root is linked under the
+// smaller one with a compare-and-swap, retrying from the current roots when
+// another thread got there first, so parent[x] <= x always holds and every
+// tree's root is its minimum vertex.
+__device__ __forceinline__ void wcc_link(int32_t* parent, int32_t a, int32_t b) {
+  int32_t ra = find_root(parent, a), rb = find_root(parent, b);
+  while (ra != rb) {
+    const int32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+    const int32_t old = atomicCAS(&parent[hi], hi, lo);
+    if (old == hi) break;
+    // hi stopped being a root: continue from its current ancestors.
+    ra = find_root(parent, old);
+    rb = find_root(parent, lo);
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_wcc_hook(const int32_t* rows, const int32_t* idx, int64_t E,
                                                     int32_t* parent, int half) {
   for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * BLOCK) {
     const int32_t a = rows[e], b = idx[e];
     if (a == b || (half && a < b)) continue;  // symmetric: each undirected edge once
-    int32_t ra = find_root(parent, a), rb = find_root(parent, b);
-    while (ra != rb) {
-      const int32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
-      const int32_t old = atomicCAS(&parent[hi], hi, lo);
-      if (old == hi) break;
-      // hi stopped being a root: continue from its current ancestors.
-      ra = find_root(parent, old);
-      rb = find_root(parent, lo);
-    }
+    wcc_link(parent, a, b);
+  }
+}
+
+// Afforest-style sampling (Sutton, Ben-Nun, Barak, "Optimizing Parallel
+// Graph Connectivity Computation via Subgraph Sampling", IPDPS 2018): link
+// every vertex with its first `k` neighbours.
+__global__ __launch_bounds__(BLOCK) void k_wcc_sample(const int32_t* off, const int32_t* idx, int32_t V,
+                                                      int k, int32_t* parent) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t e0 = off[v], e1 = min(off[v + 1], e0 + k);
+    for (int32_t e = e0; e < e1; ++e) wcc_link(parent, (int32_t)v, idx[e]);
+  }
+}
+
+// The remaining edges (from the k-th on) of every vertex outside the
+// component rooted at `skip`.  On a symmetric graph an edge between that
+// component and another vertex is also stored at the other vertex, which
+// processes it, so the skipped vertices lose nothing.
+__global__ __launch_bounds__(BLOCK) void k_wcc_rest(const int32_t* off, const int32_t* idx, int32_t V, int k,
+                                                    int32_t skip, int32_t* parent) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t e0 = off[v] + k, e1 = off[v + 1];
+    if (e0 >= e1 || find_root(parent, (int32_t)v) == skip) continue;
+    for (int32_t e = e0; e < e1; ++e) wcc_link(parent, (int32_t)v, idx[e]);
+  }
+}
+
+// parent of n pseudo-random vertices (the host picks the most frequent root).
+__global__ void k_wcc_probe(const int32_t* parent, int32_t V, int n, int32_t* out) {
+  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  if (i < n) {
+    const u32 h = (u32)i * 2654435761u ^ 0x9E3779B9u;
+    out[i] = parent[(int32_t)(h % (u32)V)];
   }
 }
 
@@ -1299,6 +1510,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   const int bs = en->opts.block_size;
   const dim3 grid(en->grid_advance());
   launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll);
+  hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (bs == 128) {
     hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
   } else if (bs == 512) {
@@ -1356,6 +1568,21 @@ void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* src, int n
   }
 }
 
+u32 narrow_mask(const Cfg& cfg, int W) {
+  // Words whose SSSP slots all sit in lanes 0..15 can relax four edges per
+  // lane-per-query step (phase N).  Measured on RMAT-22: worth it when every
+  // SSSP word of the pass is narrow (16 SSSP queries: 55 -> 36 ms), a loss
+  // when a full SSSP word runs beside the narrow one (80 SSSP: 86 -> 94 ms),
+  // so the mask is all or nothing.
+  u32 m = 0;
+  for (int w = 0; w < W; ++w) {
+    if (!cfg.val_bits[w]) continue;
+    if (cfg.val_bits[w] >> 16) return 0u;
+    m |= 1u << w;
+  }
+  return m;
+}
+
 void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
   if (n == 0) return;
   size_t blocks = std::min<size_t>((n / 4 + BLOCK - 1) / BLOCK + 1, (size_t)cus * 16);
@@ -1368,11 +1595,43 @@ static int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
 void compute_wcc(cgp_engine* en) {
   const cgp_graph* g = en->g;
   en->parent.ensure((size_t)g->V * 4 + 4);
+  int32_t* parent = (int32_t*)en->parent.p;
   const int gb = en->cus * 16;
-  hipLaunchKernelGGL(k_wcc_init, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, g->V);
-  hipLaunchKernelGGL(k_wcc_hook, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->rows,
-                     (const int32_t*)g->idx, g->E, (int32_t*)en->parent.p, g->symmetric ? 1 : 0);
-  hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, (int32_t*)en->parent.p, g->V);
+  hipLaunchKernelGGL(k_wcc_init, dim3(gb), dim3(BLOCK), 0, en->s, parent, g->V);
+  if (!g->symmetric || g->V < 4096) {
+    // Directed graphs (weak components need every stored edge) and tiny
+    // graphs: one hooking pass over all edges.
+    hipLaunchKernelGGL(k_wcc_hook, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->rows,
+                       (const int32_t*)g->idx, g->E, parent, g->symmetric ? 1 : 0);
+  } else {
+    // Sample two neighbours per vertex, find the component most vertices
+    // joined, then only the vertices outside it link their other edges.
+    constexpr int kSample = 2, kProbe = 1024;
+    hipLaunchKernelGGL(k_wcc_sample, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->off,
+                       (const int32_t*)g->idx, g->V, kSample, parent);
+    hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, parent, g->V);
+    en->wcc_probe.ensure(kProbe * 4);
+    hipLaunchKernelGGL(k_wcc_probe, dim3(kProbe / BLOCK), dim3(BLOCK), 0, en->s, (const int32_t*)parent, g->V,
+                       kProbe, (int32_t*)en->wcc_probe.p);
+    std::vector<int32_t> probe(kProbe);
+    HIP_CHECK(hipMemcpyAsync(probe.data(), en->wcc_probe.p, kProbe * 4, hipMemcpyDeviceToHost, en->s));
+    HIP_CHECK(hipStreamSynchronize(en->s));
+    std::sort(probe.begin(), probe.end());
+    int32_t skip = probe[0];
+    int best = 0;
+    for (int i = 0; i < kProbe;) {
+      int j = i;
+      while (j < kProbe && probe[j] == probe[i]) ++j;
+      if (j - i > best) {
+        best = j - i;
+        skip = probe[i];
+      }
+      i = j;
+    }
+    hipLaunchKernelGGL(k_wcc_rest, dim3(gb), dim3(BLOCK), 0, en->s, (const int32_t*)g->off,
+                       (const int32_t*)g->idx, g->V, kSample, skip, parent);
+  }
+  hipLaunchKernelGGL(k_wcc_compress, dim3(gb), dim3(BLOCK), 0, en->s, parent, g->V);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1529,6 +1788,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
         cfg.val_bits[w] |= 1ull << q;
       }
     }
+    cfg.narrow_words = rshift >= 4 && cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, W) : 0u;
     // Bottom-up steps pay off only when few BFS queries share the words: with
     // many, the missing bits of a vertex are rarely all found early.  Hybrid
     // mode pulls only for batches of at most pull_max_queries BFS queries
@@ -1604,7 +1864,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->stamp_epoch += hc.iter + 2;
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
     en->tick(CAT_OUT, true);
-    hipLaunchKernelGGL(k_output, dim3((V + 63) / 64, Wused), dim3(BLOCK), 0, en->s,
+    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK), 0, en->s,
                        (const u32*)en->vals.p, V, rshift, (const int32_t*)en->qmap.p, dout);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
@@ -1790,6 +2050,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->dense_threshold = 8;
   o->frontier_dense_div = 8;
   o->dense_unroll = 8;
+  o->dense_narrow = 1;
   o->sssp_delta = -1;
   o->graph_iterations = 8;
   o->profile = 0;
@@ -1867,7 +2128,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   en->drop_graphs();
   DevBuf* bufs[] = {&en->visited, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
-                    &en->large[1], &en->dense[0], &en->dense[1], &en->parent, &en->wcc_rows,
+                    &en->large[1], &en->dense[0], &en->dense[1], &en->dense_n[0], &en->dense_n[1], &en->parent, &en->wcc_rows, &en->wcc_probe,
                     &en->outbuf, &en->qmap, &en->slot_src, &en->iters};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);

@@ -102,10 +102,10 @@ struct cgp_engine {
   hipStream_t s = nullptr;
   int cus = 1;
   int wcap = 0;            // words allocated in visited/next/front/dense
-  cgpi::DevBuf visited, front, mark, vals, touched, small[2], medium[2], large[2], dense[2];  // visited: visited + next records
+  cgpi::DevBuf visited, front, mark, vals, touched, small[2], medium[2], large[2], dense[2], dense_n[2];  // visited: visited + next records
   cgpi::DevBuf iters;  // per-iteration record ring (ITER_RING entries)
   std::vector<cgpk::IterRec> report;  // records of the last batch run, all passes
-  cgpi::DevBuf parent, wcc_rows, outbuf, qmap, slot_src;
+  cgpi::DevBuf parent, wcc_rows, wcc_probe, outbuf, qmap, slot_src;
   Ctl* d_ctl = nullptr;
   Cfg* d_cfg = nullptr;
   Ctl* h_ctl = nullptr;     // pinned
@@ -161,6 +161,7 @@ struct cgp_engine {
       P.medium[k] = (int32_t*)medium[k].p;
       P.large[k] = (u64*)large[k].p;
       P.dense[k] = (u64*)dense[k].p;
+      P.dense_n[k] = (u64*)dense_n[k].p;
     }
     P.ctl = d_ctl;
     P.cfg = d_cfg;
@@ -203,7 +204,10 @@ struct cgp_engine {
       moved |= visited.ensure(V * W * 16);  // visited + next records
       moved |= front.ensure(V * W * 8);
       HIP_CHECK(hipMemsetAsync(visited.p, 0, V * W * 16, s));
-      for (int k = 0; k < 2; ++k) moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
+      for (int k = 0; k < 2; ++k) {
+        moved |= dense[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
+        moved |= dense_n[k].ensure((size_t)W * (V + E / DCHUNK + 1) * 8);
+      }
       wcap = W;
     }
     if (!mark.p) {
@@ -256,6 +260,8 @@ void dispatch_iteration(cgp_engine* en, int W, const KP& P);
 void dispatch_seed_only(cgp_engine* en, int W, const KP& P, const int32_t* d_slot_src, int nslots);
 void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* d_slot_src, int nslots);
 void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus);
+// Narrow-word mask of a slot configuration (all SSSP words narrow, else 0).
+u32 narrow_mask(const Cfg& cfg, int W);
 // One batched run (all passes, WCC, output transpose); stats may be null.
 void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out, bool out_dev,
                       cgp_run_stats* stats);

@@ -16,7 +16,7 @@
 //   mark  uint32[V]                    iteration stamp: vertex already appended
 //                                      to the touched list this iteration
 //   touched int32[V]                   vertices whose next words gained bits
-//   small/medium int32[V], large uint64[V+E/LCHUNK+1], dense uint64[W*(V+E/DCHUNK+1)]
+//   small/medium int32[V], large uint64[V+E/LCHUNK+1], dense and dense_n uint64[W*(V+E/DCHUNK+1)]
 //                                      degree-binned frontier work lists,
 //                                      double-buffered by iteration parity
 //
@@ -58,7 +58,8 @@ struct Cfg {
   int32_t dense_t;      // popcount at which an SSSP word goes lane-per-query
   u32 delta;            // near-far step: thr moves to far_min + delta when no pair is below
                         // it (0: no postponing)
-  int32_t pad_;
+  u32 narrow_words;     // bit w: every SSSP slot of word w is in lanes 0..15 (and R >= 16), so
+                        // its dense items relax four edges per wave step
   u64 pull_pairs_thr;   // hybrid: pull when BFS frontier (vertex,query) pairs >= this
   u64 pull_edges_thr;   // ... or when their edge-weighted count >= this
   u64 push_pairs_thr;   // back to push when pairs fall below this
@@ -84,8 +85,8 @@ struct Ctl {
   u32 near_new;     // nonzero when this iteration produced an SSSP pair below thr
   u32 bumps;        // threshold raises so far
   u32 pad;
-  u32 bcount[2][4];           // [parity][small, medium, large items, dense items]
-  u32 fetch[2][5];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high)
+  u32 bcount[2][5];           // [parity][small, medium, large items, dense items, narrow dense items]
+  u32 fetch[2][6];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high, N)
   u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
   u64 bfs_edges[2];           // ... weighted by degree
   u64 fedges[2];              // summed degree of the next frontier's vertices
@@ -139,6 +140,7 @@ struct KP {
   int32_t* medium[2];
   u64* large[2];
   u64* dense[2];
+  u64* dense_n[2];  // lane-per-query items of narrow words (SSSP slots all in lanes 0..15)
   Ctl* ctl;
   const Cfg* cfg;
   IterRec* iters;

@@ -210,6 +210,7 @@ void admit(cgp_engine* en, ServeState* ss) {
     any = true;
   }
   if (!any) return;
+  cfg.narrow_words = cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, ss->W) : 0u;
   HIP_CHECK(hipMemcpyAsync(en->d_cfg, en->h_cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipMemcpyAsync(ss->d_src.p, h_src, (size_t)ss->nslots * 4, hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipEventRecord(ss->staged, en->s));

@@ -89,6 +89,9 @@ typedef struct {
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
   int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4, 8
                                   or 16 (fewer registers, more waves vs more rows per wave); 0 = 8 */
+  int32_t dense_narrow;        /* 1 = when every SSSP word of a pass has its SSSP slots in lanes
+                                  0..15, those words relax four edges per lane-per-query step
+                                  (own item list and launch); 0 = off */
   int32_t pull_max_queries;    /* hybrid mode takes bottom-up steps only for batches of at most
                                   this many BFS queries (0 = never; CGP_PULL always pulls) */
   int32_t advance_blocks;      /* workgroups of the advance launches (256-thread equivalents);

@@ -157,6 +157,22 @@ def test_mixed_lane_per_query_variants(gpu, unroll, dense_t, delta):
     assert np.array_equal(out, oracle_rows(og, kinds, sources))
 
 
+@pytest.mark.parametrize("div", [1, 1000000])
+@pytest.mark.parametrize("n_sssp,n_bfs", [(16, 0), (12, 100), (16, 48)])
+def test_narrow_lane_per_query_words(gpu, div, n_sssp, n_bfs):
+    # SSSP slots confined to lanes 0..15 of one word: four edges per wave step
+    # (phase N), in the sparse (div 1) and dense (div 1e6) frontier modes.
+    say("narrow", n_sssp, n_bfs, "div", div)
+    h, g, og = rmat(12, 16, 7, 1, 255)
+    src = h.sample_sources(n_sssp + n_bfs, seed=div % 97)
+    kinds = [SSSP] * n_sssp + [BFS] * n_bfs
+    en = cgp_amd.Engine(g, dense_threshold=1, frontier_dense_div=div)
+    out, _ = en.run_batch(kinds, src)
+    assert np.array_equal(out, oracle_rows(og, kinds, src))
+    off, _ = cgp_amd.Engine(g, dense_threshold=1, frontier_dense_div=div, dense_narrow=0).run_batch(kinds, src)
+    assert np.array_equal(out, off)
+
+
 @pytest.mark.parametrize("delta", [1, 7, 3000])
 @pytest.mark.parametrize("nq", [1, 64])
 def test_near_far_sssp_rmat(gpu, delta, nq):
