@@ -47,6 +47,7 @@ class cgp_engine_opts(C.Structure):
         ("frontier_dense_div", C.c_int32),
         ("dense_unroll", C.c_int32),
         ("dense_narrow", C.c_int32),
+        ("concurrent_advance", C.c_int32),
         ("pull_max_queries", C.c_int32),
         ("advance_blocks", C.c_int32),
         ("sssp_delta", C.c_int32),

@@ -1509,14 +1509,29 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   hipStream_t s = en->s;
   const int bs = en->opts.block_size;
   const dim3 grid(en->grid_advance());
+  // The lane-per-query launches and the push launch touch disjoint work
+  // (atomics only on shared words), so with concurrent_advance the push
+  // launch runs on a second stream forked from and joined back into the
+  // engine stream (also inside a captured graph): each fills the other's
+  // tail.
+  hipStream_t sp = s;
+  if (en->opts.concurrent_advance) {
+    HIP_CHECK(hipEventRecord(en->ev_fork, s));
+    HIP_CHECK(hipStreamWaitEvent(en->s2, en->ev_fork, 0));
+    sp = en->s2;
+  }
+  if (bs == 128) {
+    hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, sp, P);
+  } else if (bs == 512) {
+    hipLaunchKernelGGL((k_adv_push<W, 512>), grid, dim3(512), 0, sp, P);
+  } else {
+    hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, sp, P);
+  }
   launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll);
   hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
-  if (bs == 128) {
-    hipLaunchKernelGGL((k_adv_push<W, 128>), grid, dim3(128), 0, s, P);
-  } else if (bs == 512) {
-    hipLaunchKernelGGL((k_adv_push<W, 512>), grid, dim3(512), 0, s, P);
-  } else {
-    hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, s, P);
+  if (en->opts.concurrent_advance) {
+    HIP_CHECK(hipEventRecord(en->ev_join, en->s2));
+    HIP_CHECK(hipStreamWaitEvent(s, en->ev_join, 0));
   }
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
   HIP_CHECK(hipGetLastError());
@@ -2051,6 +2066,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->frontier_dense_div = 8;
   o->dense_unroll = 8;
   o->dense_narrow = 1;
+  o->concurrent_advance = 1;
   o->sssp_delta = -1;
   o->graph_iterations = 8;
   o->profile = 0;
@@ -2111,6 +2127,9 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
         en->clock_khz = (double)khz;
     }
     HIP_CHECK(hipStreamCreateWithFlags(&en->s, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&en->s2, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&en->ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&en->ev_join, hipEventDisableTiming));
     HIP_CHECK(hipMalloc((void**)&en->d_ctl, sizeof(Ctl)));
     HIP_CHECK(hipMalloc((void**)&en->d_cfg, sizeof(Cfg)));
     HIP_CHECK(hipHostMalloc((void**)&en->h_ctl, sizeof(Ctl), hipHostMallocDefault));
@@ -2137,6 +2156,9 @@ void cgp_engine_destroy(cgp_engine* en) {
   if (en->h_ctl) (void)hipHostFree(en->h_ctl);
   if (en->h_cfg) (void)hipHostFree(en->h_cfg);
   if (en->h_stage) (void)hipHostFree(en->h_stage);
+  if (en->ev_fork) (void)hipEventDestroy(en->ev_fork);
+  if (en->ev_join) (void)hipEventDestroy(en->ev_join);
+  if (en->s2) (void)hipStreamDestroy(en->s2);
   if (en->s) (void)hipStreamDestroy(en->s);
   delete en;
 }

@@ -100,6 +100,8 @@ struct cgp_engine {
   const cgp_graph* g = nullptr;
   cgp_engine_opts opts{};
   hipStream_t s = nullptr;
+  hipStream_t s2 = nullptr;  // second stream of a concurrent advance (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cus = 1;
   int wcap = 0;            // words allocated in visited/next/front/dense
   cgpi::DevBuf visited, front, mark, vals, touched, small[2], medium[2], large[2], dense[2], dense_n[2];  // visited: visited + next records

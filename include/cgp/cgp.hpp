@@ -162,6 +162,7 @@ struct run_options {
   int frontier_dense_div = 0;
   int dense_unroll = 0;
   int dense_narrow = 1;
+  int concurrent_advance = 1;
   int pull_max_queries = 0;
   int advance_blocks = 0;
   int sssp_delta = -1;  // -1: automatic (engine default)
@@ -186,6 +187,7 @@ struct run_options {
     if (dense_unroll) o.dense_unroll = dense_unroll;
     o.pull_max_queries = pull_max_queries;
     o.dense_narrow = dense_narrow;
+    o.concurrent_advance = concurrent_advance;
     if (advance_blocks) o.advance_blocks = advance_blocks;
     o.sssp_delta = sssp_delta;
     return o;
