@@ -455,7 +455,7 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
 // address and discard it) so the U source rows and U target rows of a step
 // are all in flight together, and the loads of step k+1 are issued before
 // the atomics of step k, so waiting for them never waits on the atomics.
-template <int W, bool SPARSE, int U>
+template <int W, bool SPARSE, int U, bool NF>
 __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
                                                   u32 thr, Tally& t) {
   const int lane = lane_id();
@@ -483,7 +483,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   for (int ww = 0; ww < W; ++ww) lane_words |= (u32)((P.cfg->val_bits[ww] >> lq) & 1ull) << ww;
   if (!inrow) lane_words = 0;
   const u32 lq4 = (u32)lq * 4u;
-  if (thr != INF) {
+  if (NF) {
     // Near-far split of the wave's items: lane q reads value q of each
     // item's source row (8 rows in flight); pairs at or above thr are
     // postponed (chunk 0 of the vertex keeps them in the next frontier) and
@@ -561,7 +561,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
     // Shuffles run with every lane active (a lane past the end may still be
     // the source of another lane's item), then the select.
     const u32 am_lo = (u32)__shfl((int)smlo, j, 64), am_hi = (u32)__shfl((int)smhi, j, 64);
-    const u64 my_am = ok ? ((((u64)am_hi << 32) | am_lo) & row_lanes) : 0ull;
+    const u64 my_am = ok ? ((((u64)am_hi << 32) | am_lo) & row_lanes) : 0ull;  // read only with NF
     const int cnt = (int)min(64u, total - win);
     // Per-step state: uniform row pointers, weight and active mask; per
     // lane the two loaded values.
@@ -576,10 +576,21 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
         const u64 ua = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_uaddr >> 32), jj) << 32) |
                        (u32)__builtin_amdgcn_readlane((int)(u32)my_uaddr, jj);
         cand[k] = (u32)__builtin_amdgcn_readlane((int)myw, jj);
-        am[k] = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_am >> 32), jj) << 32) |
-                (u32)__builtin_amdgcn_readlane((int)(u32)my_am, jj);
         const u32 wk = (u32)__builtin_amdgcn_readlane(myword, jj);
-        const u32 off = ((lane_words >> wk) & 1u) ? lq4 : 0u;
+        const bool inw = ((lane_words >> wk) & 1u) != 0;
+        if (NF) {
+          am[k] = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(my_am >> 32), jj) << 32) |
+                  (u32)__builtin_amdgcn_readlane((int)(u32)my_am, jj);
+        } else {
+          // Without postponing every SSSP lane of the word is compared: a
+          // query not in u's frontier has relaxed u's edges with u's current
+          // value already, so its compare succeeds only if u improved during
+          // this iteration, which is a real path (its next bit is then set).
+          // Steps past the window's last edge read a dummy row and are
+          // masked out entirely.
+          am[k] = __ballot(inw) & (k0 + k < cnt ? ~0ull : 0ull);
+        }
+        const u32 off = inw ? lq4 : 0u;
         du[k] = *(const gu32*)(ua + off);
         cur[k] = *(const gu32*)(vr[k] + off);
       }
@@ -999,7 +1010,8 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   while (true) {
     const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
-    expand_flat_dense<W, SPARSE, U>(P, i, nD, rshift, stamp, thr, t);
+    if (thr == INF) expand_flat_dense<W, SPARSE, U, false>(P, i, nD, rshift, stamp, thr, t);
+    else expand_flat_dense<W, SPARSE, U, true>(P, i, nD, rshift, stamp, thr, t);
   }
   flush_near_far(P, t, thr);
   if (P.profile) {
