@@ -93,8 +93,8 @@ typedef struct {
                                   0..15, those words relax four edges per lane-per-query step
                                   (own item list and launch); 0 = off */
   int32_t concurrent_advance;  /* 1 = the push launch runs beside the lane-per-query launches on
-                                  a second stream (fork/join events, also inside graphs); 0 = one
-                                  stream */
+                                  a second stream (fork/join events, also inside graphs); 0
+                                  (default) = one stream */
   int32_t pull_max_queries;    /* hybrid mode takes bottom-up steps only for batches of at most
                                   this many BFS queries (0 = never; CGP_PULL always pulls) */
   int32_t advance_blocks;      /* workgroups of the advance launches (256-thread equivalents);

@@ -162,7 +162,7 @@ struct run_options {
   int frontier_dense_div = 0;
   int dense_unroll = 0;
   int dense_narrow = 1;
-  int concurrent_advance = 1;
+  int concurrent_advance = 0;
   int pull_max_queries = 0;
   int advance_blocks = 0;
   int sssp_delta = -1;  // -1: automatic (engine default)
