@@ -31,6 +31,18 @@
 using namespace cgpk;
 using namespace cgpi;
 
+// AMDGPU intrinsics without a clang builtin, bound by their LLVM names:
+// raw buffer load and atomicMin (address = base + voffset + soffset, with
+// the 32-bit offsets range-checked against the descriptor) and v_writelane.
+// (Struct-addressed buffer loads, vindex * stride + voffset, measured 2.4x
+// slower on random 256-byte row gathers than raw or global loads.)
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+extern "C" __device__ int cgp_buf_load_i32(i32x4 rsrc, int voffset, int soffset,
+                                           int aux) __asm("llvm.amdgcn.raw.buffer.load.i32");
+extern "C" __device__ int cgp_buf_atomic_umin_i32(int vdata, i32x4 rsrc, int voffset, int soffset,
+                                                  int aux) __asm("llvm.amdgcn.raw.buffer.atomic.umin.i32");
+extern "C" __device__ int cgp_writelane_i32(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 // ============================================================== device helpers
 namespace {
 
@@ -145,6 +157,17 @@ __device__ __forceinline__ gu32* uniform_gptr(const u32* p) {
   const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)a);
   const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(a >> 32));
   return (gu32*)(((u64)hi << 32) | lo);
+}
+
+// Raw buffer resource over `bytes` bytes from `base` (bytes < 2^32).  The
+// fields are wave-uniform so the descriptor stays in scalar registers.
+__device__ __forceinline__ i32x4 raw_rsrc(u64 base, u32 bytes) {
+  i32x4 r;
+  r.x = __builtin_amdgcn_readfirstlane((int)(u32)base);
+  r.y = __builtin_amdgcn_readfirstlane((int)((u32)(base >> 32) & 0xFFFFu));  // stride 0
+  r.z = __builtin_amdgcn_readfirstlane((int)bytes);
+  r.w = 0x00020000;  // 32-bit data format: untyped accesses need a valid one
+  return r;
 }
 
 __device__ __forceinline__ size_t vrow(int32_t V, int w, int32_t v, int rshift) {
@@ -644,6 +667,128 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   }
 }
 
+// Phase D without postponing, for value rows of one word spanning less than
+// 4 GiB.  The wave's items are taken one word at a time, so a window's edges
+// all belong to one word: its active-lane mask, lane offsets and buffer
+// descriptor (based at the word's first row) are fixed for the window, and
+// row addresses are 32-bit byte offsets.  Once per window lane f writes edge
+// f's target and source row offsets and weight into the wave's LDS table;
+// each step reads its entry back as a broadcast LDS load (no per-step scalar
+// work), adds the lane offset, reads the two rows with raw buffer loads, and
+// the improved lanes drive the buffer atomicMin directly as the exec mask.
+// In dense frontier mode lane k collects step k's improved mask with
+// v_writelane, and one atomicOr per lane per window publishes the next bits.
+// Steps past the window's last edge read an entry of weight CGP_INF, so they
+// cannot improve anything.
+template <int W, bool SPARSE, int U>
+__device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
+                                                      uint4* tab, Tally& t) {
+  const int lane = lane_id();
+  const int R = 1 << rshift;
+  const u32 p = P.ctl->iter & 1u;
+  const bool have = base + (u32)lane < n;
+  const u64 item = have ? P.dense[p][base + lane] : 0ull;
+  const int32_t u = (int32_t)(item >> 32);
+  const int w = (int)((item >> 24) & 0xFF);
+  const u32 chunk = (u32)(item & 0xFFFFFF);
+  const int32_t e0 = have ? P.off[u] + (int32_t)(chunk * DCHUNK) : 0;
+  const int32_t e1 = have ? min(P.off[u + 1], e0 + DCHUNK) : 0;
+  const u64 sm = have ? (P.front[(size_t)u * W + w] & P.cfg->val_bits[w]) : 0ull;
+  const u32 dlen = sm ? (u32)(e1 - e0) : 0u;
+  if (have) {
+    ++t.ditems;
+    t.dedges += dlen;
+    t.sssp += (u64)dlen * (u64)__popcll(sm);
+  }
+  const bool inrow = lane < R;
+  const int lq = lane & (R - 1);
+  const u32 lq4 = (u32)lq * 4u;
+  const u32 rowb = 4u << rshift;               // bytes per value row
+  const u32 wordb = (u32)P.V * rowb;           // bytes per word (< 2^32, checked by the caller)
+  const u32 srcb = (u32)u * rowb;              // the item's source row inside its word
+  u32 imp_n = 0;  // improved (vertex, query) pairs, wave-uniform
+  for (int wp = 0; wp < W; ++wp) {
+    const u64 vb = P.cfg->val_bits[wp];
+    if (vb == 0 || __ballot(dlen != 0 && w == wp) == 0) continue;
+    const i32x4 rs = raw_rsrc((u64)P.vals + (u64)wp * wordb, wordb);
+    // This word's SSSP lanes read their own value; the others read the
+    // row's first value (the same lines) and are masked out.
+    const bool inw = inrow && ((vb >> lq) & 1ull) != 0;
+    const u64 am = __ballot(inw);
+    const u32 off = inw ? lq4 : 0u;
+    const u32 d = w == wp ? dlen : 0u;
+    const u32 incl = wave_incl_scan(d);
+    const u32 total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
+    for (u32 win = 0; win < total; win += 64) {
+      const u32 f = win + (u32)lane;
+      const int j = wave_upper_bound(incl, f);
+      const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
+      const int32_t ej = __shfl(e0, j, 64) + (int32_t)(f - exj);
+      const u32 sb = (u32)__shfl((int)srcb, j, 64);
+      const bool ok = f < total;
+      const int32_t myv = ok ? P.idx[ej] : 0;
+      const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : INF;
+      tab[lane] = make_uint4((u32)myv * rowb, ok ? sb : 0u, myw, 0u);
+      __builtin_amdgcn_wave_barrier();
+      const int cnt = (int)min(64u, total - win);
+      u32 ot[U], wt[U], du[U], cur[U];
+      auto issue = [&](int k0) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const uint4 e = tab[k0 + k];  // k0 + k < 64: k0 < cnt <= 64 and U divides 64
+          ot[k] = e.x + off;
+          wt[k] = e.z;
+          du[k] = (u32)cgp_buf_load_i32(rs, (int)(e.y + off), 0, 0);
+          cur[k] = (u32)cgp_buf_load_i32(rs, (int)ot[k], 0, 0);
+        }
+      };
+      issue(0);
+      u32 mine_lo = 0, mine_hi = 0;
+      for (int k0 = 0; k0 < cnt; k0 += U) {
+        u64 mk[U];
+        u32 c_c[U], ot_c[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          // Saturating add: a path past CGP_INF stays unreachable.
+          const u32 c = __builtin_elementwise_add_sat(du[k], wt[k]);
+          mk[k] = __ballot(c < cur[k]) & am;
+          c_c[k] = c;
+          ot_c[k] = ot[k];
+        }
+        if (k0 + U < cnt) issue(k0 + U);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const u64 m = mk[k];
+          if (m == 0) continue;
+          imp_n += (u32)__popcll(m);
+          // The improved lanes are SSSP lanes of the word, whose offset is
+          // their own value's.
+          if (__builtin_amdgcn_inverse_ballot_w64(m)) cgp_buf_atomic_umin_i32((int)c_c[k], rs, (int)ot_c[k], 0, 0);
+          if (!SPARSE) {
+            mine_lo = (u32)cgp_writelane_i32((int)(u32)m, k0 + k, (int)mine_lo);
+            mine_hi = (u32)cgp_writelane_i32((int)(u32)(m >> 32), k0 + k, (int)mine_hi);
+          } else {
+            const int32_t vk = __builtin_amdgcn_readlane(myv, k0 + k);
+            bool claimed = false;
+            if (lane == 0) {
+              ++t.nnew;
+              if (atomicOr(&P.next[nxt_at<W>(vk, wp)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
+            }
+            wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
+          }
+        }
+      }
+      // Dense mode: lane f owns edge f's next-word update.
+      if (!SPARSE && (mine_lo | mine_hi)) {
+        ++t.nnew;
+        atomicOr(&P.next[nxt_at<W>(myv, wp)], ((u64)mine_hi << 32) | mine_lo);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (lane == 0) t.imp += imp_n;
+}
+
 // Lane K of each 16-lane row, broadcast to the whole row (DPP row_newbcast).
 template <int K>
 __device__ __forceinline__ u32 row_bcast(u32 x) {
@@ -989,7 +1134,7 @@ __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
 // register budget, one instantiation per frontier mode (the one that does
 // not match the device-selected mode exits at once); k_adv_push runs phases
 // L, M, S and P.
-template <int W, int BS, bool SPARSE, int U>
+template <int W, int BS, bool SPARSE, int U, bool TAB>
 __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   Ctl* C = P.ctl;
   if (C->finished || (C->dense_mode == 0) != SPARSE) return;
@@ -1005,12 +1150,18 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   const u32 thr = C->thr;
   const int rshift = P.cfg->rshift;
   Tally t;
+  // TAB: postponing off (thr is CGP_INF throughout) and one word's value
+  // rows span less than 4 GiB (32-bit buffer offsets); a kernel of its own,
+  // so the other loops' registers do not lower its occupancy.
+  __shared__ uint4 s_tab[TAB ? BS : 1];  // per wave: 64 step entries of the current window
+  uint4* tab = s_tab + (TAB ? (threadIdx.x & ~63u) : 0u);
   // One group of 64 items per fetch: items carry up to DCHUNK edges each, so
   // coarser claims leave a long tail (measured 2.5x slower at 4 groups).
   while (true) {
     const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
-    if (thr == INF) expand_flat_dense<W, SPARSE, U, false>(P, i, nD, rshift, stamp, thr, t);
+    if (TAB) expand_flat_dense_tab<W, SPARSE, U>(P, i, nD, rshift, stamp, tab, t);
+    else if (thr == INF) expand_flat_dense<W, SPARSE, U, false>(P, i, nD, rshift, stamp, thr, t);
     else expand_flat_dense<W, SPARSE, U, true>(P, i, nD, rshift, stamp, thr, t);
   }
   flush_near_far(P, t, thr);
@@ -1503,17 +1654,28 @@ namespace {
 // not matching the device-selected mode exits at once), U = rows per step.
 // It is wave-level work, so it always runs 256-thread workgroups.
 template <int W, int U>
-void launch_dense_u(hipStream_t s, dim3 grid, const KP& P) {
-  hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U>), grid, dim3(BLOCK), 0, s, P);
-  hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U>), grid, dim3(BLOCK), 0, s, P);
+void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, bool tab) {
+  if (tab) {
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, true>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, true>), grid, dim3(BLOCK), 0, s, P);
+  } else {
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, false>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, false>), grid, dim3(BLOCK), 0, s, P);
+  }
+}
+
+// The table loop needs postponing off and 32-bit byte offsets inside one
+// word's value rows at the widest row (64 slots).
+bool dense_tab_ok(const cgp_engine* en) {
+  return en->opts.sssp_delta == 0 && (u64)en->g->V * 256ull < (1ull << 32);
 }
 
 template <int W>
-void launch_dense(hipStream_t s, int blocks, const KP& P, int unroll) {
+void launch_dense(hipStream_t s, int blocks, const KP& P, int unroll, bool tab) {
   const dim3 grid((unsigned)blocks);
-  if (unroll == 4) launch_dense_u<W, 4>(s, grid, P);
-  else if (unroll == 16) launch_dense_u<W, 16>(s, grid, P);
-  else launch_dense_u<W, 8>(s, grid, P);
+  if (unroll == 4) launch_dense_u<W, 4>(s, grid, P, tab);
+  else if (unroll == 16) launch_dense_u<W, 16>(s, grid, P, tab);
+  else launch_dense_u<W, 8>(s, grid, P, tab);
 }
 
 // One iteration: advance then filter (both time themselves on the device clock).
@@ -1540,7 +1702,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   } else {
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, sp, P);
   }
-  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll);
+  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_ok(en));
   hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (en->opts.concurrent_advance) {
     HIP_CHECK(hipEventRecord(en->ev_join, en->s2));
@@ -2077,7 +2239,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->use_hip_graph = 1;
   o->dense_threshold = 8;
   o->frontier_dense_div = 8;
-  o->dense_unroll = 8;
+  o->dense_unroll = 16;
   o->dense_narrow = 1;
   o->concurrent_advance = 0;
   o->sssp_delta = -1;

@@ -88,7 +88,7 @@ typedef struct {
   int32_t frontier_dense_div;  /* dense frontier mode (fire-and-forget atomics, filter scans all
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
   int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4, 8
-                                  or 16 (fewer registers, more waves vs more rows per wave); 0 = 8 */
+                                  or 16 (fewer registers, more waves vs more rows per wave); 0 = 16 */
   int32_t dense_narrow;        /* 1 = when every SSSP word of a pass has its SSSP slots in lanes
                                   0..15, those words relax four edges per lane-per-query step
                                   (own item list and launch); 0 = off */
