@@ -51,6 +51,7 @@ class cgp_engine_opts(C.Structure):
         ("pull_max_queries", C.c_int32),
         ("advance_blocks", C.c_int32),
         ("sssp_delta", C.c_int32),
+        ("dense_wide_addr", C.c_int32),
     ]
 
 

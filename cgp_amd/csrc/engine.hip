@@ -667,22 +667,23 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
   }
 }
 
-// Phase D without postponing, for value rows of one word spanning less than
-// 4 GiB.  The wave's items are taken one word at a time, so a window's edges
-// all belong to one word: its active-lane mask, lane offsets and buffer
-// descriptor (based at the word's first row) are fixed for the window, and
-// row addresses are 32-bit byte offsets.  Once per window lane f writes edge
-// f's target and source row offsets and weight into the wave's LDS table;
-// each step reads its entry back as a broadcast LDS load (no per-step scalar
-// work), adds the lane offset, reads the two rows with raw buffer loads, and
-// the improved lanes drive the buffer atomicMin directly as the exec mask.
+// Phase D without postponing.  The wave's items are taken one word at a
+// time, so a window's edges all belong to one word: its active-lane mask and
+// lane offsets are fixed for the window.  Once per window lane f writes edge
+// f's target and source rows and weight into the wave's LDS table; each step
+// reads its entry back as a broadcast LDS load (no per-step scalar work),
+// adds the lane offset, reads the two rows, and the improved lanes drive the
+// atomicMin directly as the exec mask.  Row addresses: when one word's value
+// rows span less than 4 GiB (!A64), 32-bit byte offsets from the word's
+// first row through a raw buffer descriptor; otherwise (A64) 64-bit
+// addresses in the table and global loads.
 // In dense frontier mode lane k collects step k's improved mask with
 // v_writelane, and one atomicOr per lane per window publishes the next bits.
 // Steps past the window's last edge read an entry of weight CGP_INF, so they
 // cannot improve anything.
-template <int W, bool SPARSE, int U>
+template <int W, bool SPARSE, int U, bool A64>
 __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32 n, int rshift, u32 stamp,
-                                                      uint4* tab, Tally& t) {
+                                                      uint4* tab, u32* wtab, Tally& t) {
   const int lane = lane_id();
   const int R = 1 << rshift;
   const u32 p = P.ctl->iter & 1u;
@@ -704,8 +705,9 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
   const int lq = lane & (R - 1);
   const u32 lq4 = (u32)lq * 4u;
   const u32 rowb = 4u << rshift;               // bytes per value row
-  const u32 wordb = (u32)P.V * rowb;           // bytes per word (< 2^32, checked by the caller)
-  const u32 srcb = (u32)u * rowb;              // the item's source row inside its word
+  const u32 wordb = (u32)P.V * rowb;           // bytes per word (!A64: < 2^32, checked by the caller)
+  const u32 srcb = (u32)u * rowb;              // the item's source row inside its word (!A64)
+  const u64 srca = (u64)P.vals + ((u64)((u32)w * (u32)P.V + (u32)u) << (rshift + 2));  // (A64)
   u32 imp_n = 0;  // improved (vertex, query) pairs, wave-uniform
   for (int wp = 0; wp < W; ++wp) {
     const u64 vb = P.cfg->val_bits[wp];
@@ -724,29 +726,53 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
       const int j = wave_upper_bound(incl, f);
       const u32 exj = (u32)__shfl((int)(incl - d), j, 64);
       const int32_t ej = __shfl(e0, j, 64) + (int32_t)(f - exj);
-      const u32 sb = (u32)__shfl((int)srcb, j, 64);
       const bool ok = f < total;
       const int32_t myv = ok ? P.idx[ej] : 0;
       const u32 myw = ok ? (P.wt ? P.wt[ej] : 1u) : INF;
-      tab[lane] = make_uint4((u32)myv * rowb, ok ? sb : 0u, myw, 0u);
+      if (A64) {
+        const u64 ta = (u64)P.vals + ((u64)((u32)wp * (u32)P.V + (u32)myv) << (rshift + 2));
+        // Shuffle with every lane active (item lane j may lie past the
+        // window's last edge; a shuffle from an inactive lane reads 0), then
+        // select.
+        const u32 sa_hi = (u32)__shfl((int)(u32)(srca >> 32), j, 64);
+        const u32 sa_lo = (u32)__shfl((int)(u32)srca, j, 64);
+        const u64 sa = ok ? (((u64)sa_hi << 32) | sa_lo) : (u64)P.vals;
+        tab[lane] = make_uint4((u32)ta, (u32)(ta >> 32), (u32)sa, (u32)(sa >> 32));
+        wtab[lane] = myw;
+      } else {
+        const u32 sb = (u32)__shfl((int)srcb, j, 64);
+        tab[lane] = make_uint4((u32)myv * rowb, ok ? sb : 0u, myw, 0u);
+      }
       __builtin_amdgcn_wave_barrier();
       const int cnt = (int)min(64u, total - win);
-      u32 ot[U], wt[U], du[U], cur[U];
+      // Per step: the lane's target value (offset or address), the weight
+      // and the two loaded values.
+      typedef typename std::conditional<A64, u64, u32>::type addr_t;
+      addr_t ot[U];
+      u32 wt[U], du[U], cur[U];
       auto issue = [&](int k0) {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
           const uint4 e = tab[k0 + k];  // k0 + k < 64: k0 < cnt <= 64 and U divides 64
-          ot[k] = e.x + off;
-          wt[k] = e.z;
-          du[k] = (u32)cgp_buf_load_i32(rs, (int)(e.y + off), 0, 0);
-          cur[k] = (u32)cgp_buf_load_i32(rs, (int)ot[k], 0, 0);
+          if constexpr (A64) {
+            ot[k] = (((u64)e.y << 32) | e.x) + off;
+            wt[k] = wtab[k0 + k];
+            du[k] = *(const gu32*)((((u64)e.w << 32) | e.z) + off);
+            cur[k] = *(const gu32*)ot[k];
+          } else {
+            ot[k] = e.x + off;
+            wt[k] = e.z;
+            du[k] = (u32)cgp_buf_load_i32(rs, (int)(e.y + off), 0, 0);
+            cur[k] = (u32)cgp_buf_load_i32(rs, (int)ot[k], 0, 0);
+          }
         }
       };
       issue(0);
       u32 mine_lo = 0, mine_hi = 0;
       for (int k0 = 0; k0 < cnt; k0 += U) {
         u64 mk[U];
-        u32 c_c[U], ot_c[U];
+        u32 c_c[U];
+        addr_t ot_c[U];
 #pragma unroll
         for (int k = 0; k < U; ++k) {
           // Saturating add: a path past CGP_INF stays unreachable.
@@ -763,7 +789,12 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
           imp_n += (u32)__popcll(m);
           // The improved lanes are SSSP lanes of the word, whose offset is
           // their own value's.
-          if (__builtin_amdgcn_inverse_ballot_w64(m)) cgp_buf_atomic_umin_i32((int)c_c[k], rs, (int)ot_c[k], 0, 0);
+          if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+            if constexpr (A64)
+              __hip_atomic_fetch_min((gu32*)ot_c[k], c_c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+              cgp_buf_atomic_umin_i32((int)c_c[k], rs, (int)ot_c[k], 0, 0);
+          }
           if (!SPARSE) {
             mine_lo = (u32)cgp_writelane_i32((int)(u32)m, k0 + k, (int)mine_lo);
             mine_hi = (u32)cgp_writelane_i32((int)(u32)(m >> 32), k0 + k, (int)mine_hi);
@@ -1134,7 +1165,7 @@ __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
 // register budget, one instantiation per frontier mode (the one that does
 // not match the device-selected mode exits at once); k_adv_push runs phases
 // L, M, S and P.
-template <int W, int BS, bool SPARSE, int U, bool TAB>
+template <int W, int BS, bool SPARSE, int U, int TAB>
 __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   Ctl* C = P.ctl;
   if (C->finished || (C->dense_mode == 0) != SPARSE) return;
@@ -1150,17 +1181,19 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   const u32 thr = C->thr;
   const int rshift = P.cfg->rshift;
   Tally t;
-  // TAB: postponing off (thr is CGP_INF throughout) and one word's value
-  // rows span less than 4 GiB (32-bit buffer offsets); a kernel of its own,
-  // so the other loops' registers do not lower its occupancy.
+  // TAB (1: 32-bit row offsets, 2: 64-bit row addresses): postponing off
+  // (thr is CGP_INF throughout), the table loop in a kernel of its own so
+  // the other loops' registers do not lower its occupancy.
   __shared__ uint4 s_tab[TAB ? BS : 1];  // per wave: 64 step entries of the current window
+  __shared__ u32 s_wt[TAB == 2 ? BS : 1];
   uint4* tab = s_tab + (TAB ? (threadIdx.x & ~63u) : 0u);
+  u32* wtab = s_wt + (TAB == 2 ? (threadIdx.x & ~63u) : 0u);
   // One group of 64 items per fetch: items carry up to DCHUNK edges each, so
   // coarser claims leave a long tail (measured 2.5x slower at 4 groups).
   while (true) {
     const u32 i = wave_fetch(&C->fetch[p][1], 64u);
     if (i >= nD) break;
-    if (TAB) expand_flat_dense_tab<W, SPARSE, U>(P, i, nD, rshift, stamp, tab, t);
+    if (TAB) expand_flat_dense_tab<W, SPARSE, U, TAB == 2>(P, i, nD, rshift, stamp, tab, wtab, t);
     else if (thr == INF) expand_flat_dense<W, SPARSE, U, false>(P, i, nD, rshift, stamp, thr, t);
     else expand_flat_dense<W, SPARSE, U, true>(P, i, nD, rshift, stamp, thr, t);
   }
@@ -1654,24 +1687,29 @@ namespace {
 // not matching the device-selected mode exits at once), U = rows per step.
 // It is wave-level work, so it always runs 256-thread workgroups.
 template <int W, int U>
-void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, bool tab) {
-  if (tab) {
-    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, true>), grid, dim3(BLOCK), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, true>), grid, dim3(BLOCK), 0, s, P);
+void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, int tab) {
+  if (tab == 1) {
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, 1>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, 1>), grid, dim3(BLOCK), 0, s, P);
+  } else if (tab == 2) {
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, 2>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, 2>), grid, dim3(BLOCK), 0, s, P);
   } else {
-    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, false>), grid, dim3(BLOCK), 0, s, P);
-    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, false>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, false, U, 0>), grid, dim3(BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_adv_dense<W, BLOCK, true, U, 0>), grid, dim3(BLOCK), 0, s, P);
   }
 }
 
-// The table loop needs postponing off and 32-bit byte offsets inside one
-// word's value rows at the widest row (64 slots).
-bool dense_tab_ok(const cgp_engine* en) {
-  return en->opts.sssp_delta == 0 && (u64)en->g->V * 256ull < (1ull << 32);
+// Lane-per-query loop of an engine: the table loop when postponing is off,
+// with 32-bit byte offsets when one word's value rows (at the widest row,
+// 64 slots) span less than 4 GiB; the postponing loop otherwise.
+int dense_tab_mode(const cgp_engine* en) {
+  if (en->opts.sssp_delta != 0) return 0;
+  return !en->opts.dense_wide_addr && (u64)en->g->V * 256ull < (1ull << 32) ? 1 : 2;
 }
 
 template <int W>
-void launch_dense(hipStream_t s, int blocks, const KP& P, int unroll, bool tab) {
+void launch_dense(hipStream_t s, int blocks, const KP& P, int unroll, int tab) {
   const dim3 grid((unsigned)blocks);
   if (unroll == 4) launch_dense_u<W, 4>(s, grid, P, tab);
   else if (unroll == 16) launch_dense_u<W, 16>(s, grid, P, tab);
@@ -1702,7 +1740,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   } else {
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, sp, P);
   }
-  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_ok(en));
+  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_mode(en));
   hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (en->opts.concurrent_advance) {
     HIP_CHECK(hipEventRecord(en->ev_join, en->s2));
@@ -2243,6 +2281,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->dense_narrow = 1;
   o->concurrent_advance = 0;
   o->sssp_delta = -1;
+  o->dense_wide_addr = 0;
   o->graph_iterations = 8;
   o->profile = 0;
   o->hbm_budget_bytes = 64ull << 30;
@@ -2283,6 +2322,8 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8 && en->opts.dense_unroll != 16)
       cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4, 8 or 16");
     if (en->opts.sssp_delta < -1) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= -1");
+    if (en->opts.dense_wide_addr != 0 && en->opts.dense_wide_addr != 1)
+      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_wide_addr must be 0 or 1");
     if (en->opts.sssp_delta == -1) {
       // Automatic near-far step.  Low-degree weighted graphs (grids, road
       // networks: long shortest paths, many re-relaxations) get twenty mean

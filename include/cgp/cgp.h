@@ -105,6 +105,10 @@ typedef struct {
                                   frontier); -1 (default) = automatic: 20 x mean edge weight on
                                   weighted graphs of average degree <= 8, else 0.  Results are
                                   identical either way. */
+  int32_t dense_wide_addr;     /* 1 = the lane-per-query loop addresses value rows with 64-bit
+                                  addresses even where 32-bit offsets inside one word's rows
+                                  reach them (V x 256 bytes < 4 GiB); 0 (default) = only where
+                                  they do not */
 } cgp_engine_opts;
 
 /* Per-run report; replaces run_result_t / iteration_profile_t

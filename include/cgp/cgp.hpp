@@ -166,6 +166,7 @@ struct run_options {
   int pull_max_queries = 0;
   int advance_blocks = 0;
   int sssp_delta = -1;  // -1: automatic (engine default)
+  int dense_wide_addr = 0;
 
   cgp_engine_opts to_c() const {
     cgp_engine_opts o;
@@ -190,6 +191,7 @@ struct run_options {
     o.concurrent_advance = concurrent_advance;
     if (advance_blocks) o.advance_blocks = advance_blocks;
     o.sssp_delta = sssp_delta;
+    o.dense_wide_addr = dense_wide_addr;
     return o;
   }
 };

@@ -157,6 +157,24 @@ def test_mixed_lane_per_query_variants(gpu, unroll, dense_t, delta):
     assert np.array_equal(out, oracle_rows(og, kinds, sources))
 
 
+@pytest.mark.parametrize("unroll", [4, 16])
+@pytest.mark.parametrize("dense_t,div", [(1, 8), (8, 8), (1, 1), (1, 1000000)])
+def test_lane_per_query_wide_addresses(gpu, unroll, dense_t, div):
+    # 64-bit row addresses in the lane-per-query loop (the form used where one
+    # word's value rows pass 4 GiB), with two full SSSP words and one shared
+    # with BFS slots, in the automatic, sparse (div 1) and dense (div 1e6)
+    # frontier modes.
+    say("wide addresses unroll", unroll, "dense_t", dense_t, "div", div)
+    h, g, og = rmat(12, 16, 8, 1, 255)
+    src = h.sample_sources(240, seed=div % 89)
+    kinds = [BFS] * 100 + [SSSP] * 140 + [WCC] * 2
+    sources = list(src) + [0] * 2
+    en = cgp_amd.Engine(g, dense_unroll=unroll, dense_threshold=dense_t, frontier_dense_div=div,
+                        sssp_delta=0, dense_wide_addr=1)
+    out, _ = en.run_batch(kinds, sources)
+    assert np.array_equal(out, oracle_rows(og, kinds, sources))
+
+
 @pytest.mark.parametrize("div", [1, 1000000])
 @pytest.mark.parametrize("n_sssp,n_bfs", [(16, 0), (12, 100), (16, 48)])
 def test_narrow_lane_per_query_words(gpu, div, n_sssp, n_bfs):

@@ -46,14 +46,14 @@ def test_replenish_golden_two_slots(gpu):
     assert min(done[2]["iterations"], done[3]["iterations"]) >= 1
 
 
-@pytest.mark.parametrize("slots,delta", [(4, 0), (64, 0), (130, 0), (64, 40)])
-def test_staggered_admission_matches_oracle(gpu, slots, delta):
+@pytest.mark.parametrize("slots,delta,wide", [(4, 0, 0), (64, 0, 0), (130, 0, 0), (64, 40, 0), (64, 0, 1)])
+def test_staggered_admission_matches_oracle(gpu, slots, delta, wide):
     h = cgp_amd.HostCSR.rmat(12, 16, seed=21, wmin=1, wmax=200)
     g = cgp_amd.Graph.from_host(h)
     og = O.G.of(h)
     src = h.sample_sources(90, seed=4)
     kinds = [(BFS, SSSP, BFS)[i % 3] for i in range(90)]
-    en = cgp_amd.Engine(g, sssp_delta=delta)
+    en = cgp_amd.Engine(g, sssp_delta=delta, dense_wide_addr=wide)
     en.serve_start(max_slots=slots)
     done = {}
     nxt = 0
