@@ -293,10 +293,10 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
 }
 
 // Frontier masks of vertex u: BFS bits, and the sparse SSSP bits (popcount
-// below the word's dense threshold; denser words go lane-per-query) split at the near-far
+// below dense_t; denser words go lane-per-query) split at the near-far
 // threshold: pairs whose value is >= thr land in far[] and are not expanded.
 template <int W>
-__device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on,
+__device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, int dense_t,
                                            u32 thr, int rshift, u64 (&bm)[W], u64 (&sm)[W],
                                            u64 (&far)[W], bool& any_w, Tally& t) {
   const Cfg* cfg = P.cfg;
@@ -306,7 +306,7 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on,
     const u64 f = P.front[(size_t)u * W + w];
     bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
     u64 s = f & cfg->val_bits[w];
-    if (__popcll(s) >= cfg->dense_tw[w]) s = 0;  // handled lane-per-query
+    if (__popcll(s) >= dense_t) s = 0;  // handled lane-per-query
     far[w] = 0;
     if (s && thr != INF) {
       const size_t ub = vrow(P.V, w, u, rshift);
@@ -360,11 +360,11 @@ __device__ __forceinline__ bool postpone(const KP& P, int32_t u, const u64 (&far
 // `stride` from `first` (lane-per-edge): BFS bits and sparse SSSP words.
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0, int32_t e1,
-                                             int32_t first, int32_t stride, bool bfs_on,
+                                             int32_t first, int32_t stride, bool bfs_on, int dense_t,
                                              int rshift, u32 stamp, u32 thr, bool lead, Tally& t) {
   u64 bm[W], sm[W], far[W];
   bool any_s;
-  load_masks<W>(P, u, bfs_on, thr, rshift, bm, sm, far, any_s, t);
+  load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
   if (lead) {
     const bool c = postpone<W, SPARSE>(P, u, far, stamp, t);
     if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount, P.touched);
@@ -417,14 +417,14 @@ __device__ __forceinline__ int wave_upper_bound(u32 incl, u32 f) {
 // edges, one per lane (load-balanced lane-per-edge).
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 base, u32 n,
-                                                     bool bfs_on, int rshift, u32 stamp,
+                                                     bool bfs_on, int dense_t, int rshift, u32 stamp,
                                                      u32 thr, Tally& t) {
   const int lane = lane_id();
   const bool have = base + (u32)lane < n;
   const int32_t u = have ? list[base + lane] : 0;
   u64 bm[W], sm[W], far[W];
   bool any_s = false, any_b = false;
-  if (have) load_masks<W>(P, u, bfs_on, thr, rshift, bm, sm, far, any_s, t);
+  if (have) load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     if (!have) bm[w] = sm[w] = far[w] = 0;
@@ -1063,7 +1063,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
   const bool bfs_on = C->bfs_pull == 0;
   const u32 stamp = C->stamp_base + iter + 1;
   const u32 thr = C->thr;
-  const int rshift = P.cfg->rshift;
+  const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
   const int lane = lane_id();
   // Phase L.
   const u32 nL = C->bcount[p][2];
@@ -1077,7 +1077,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     const int32_t u = (int32_t)(item >> 32);
     const int32_t e0 = P.off[u] + (int32_t)((u32)item * LCHUNK);
     const int32_t e1 = min(P.off[u + 1], e0 + LCHUNK);
-    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, rshift,
+    expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + (int32_t)threadIdx.x, BS, bfs_on, dense_t, rshift,
                             stamp, thr, (u32)item == 0 && threadIdx.x == 0, t);
   }
   // Work cursors are single addresses every wave contends for, so each
@@ -1093,7 +1093,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     for (u32 i = i0; i < i1; ++i) {
       const int32_t u = P.medium[p][i];
       const int32_t e0 = P.off[u], e1 = P.off[u + 1];
-      expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, rshift, stamp, thr, lane == 0, t);
+      expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
     }
   }
   // Phase S.
@@ -1102,7 +1102,7 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     const u32 b0 = wave_fetch(&C->fetch[p][3], 64u * FETCH_S);
     if (b0 >= nS) break;
     for (u32 base = b0; base < nS && base < b0 + 64u * FETCH_S; base += 64u)
-      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, rshift, stamp, thr, t);
+      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, thr, t);
   }
   // Phase P: high-degree vertices one wave each (64 neighbours per step,
   // early exit on a wave-wide OR), then the rest one lane each.
@@ -1311,7 +1311,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
   const bool scan = C->dense_mode != 0;
   const u32 n = scan ? (u32)P.V : C->tcount;
-  const int rshift = cfg->rshift;
+  const int rshift = cfg->rshift, dense_t = cfg->dense_t;
   const int lane = lane_id();
   u64 bb[W], vb[W], un[W];
 #pragma unroll
@@ -1347,7 +1347,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       nb[w] = nw & bb[w];
       if (nb[w]) P.visited[vis_at<W>(v, w)] |= nb[w];
       const u64 s = nw & vb[w];
-      if (s && __popcll(s) >= cfg->dense_tw[w]) (((cfg->narrow_words >> w) & 1u) ? nmask : dmask) |= 1u << w;
+      if (s && __popcll(s) >= dense_t) (((cfg->narrow_words >> w) & 1u) ? nmask : dmask) |= 1u << w;
       else if (s) need_bin = true;
       if (nb[w]) need_bin = true;
       un[w] |= nw;
@@ -1796,20 +1796,6 @@ void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* src, int n
   }
 }
 
-// Popcount at which a vertex's SSSP pairs in word w go lane-per-query.  A
-// lane-per-query step reads the 64-byte row segments that hold the word's
-// SSSP slots, whatever the number of active pairs; a lane-per-edge
-// relaxation reads a segment per active pair.  So the threshold (dense_t for
-// a word whose SSSP slots span all four segments) scales down with the
-// segments the word's slots occupy; dense_t above 64 (never) is kept.
-void set_dense_thresholds(Cfg& cfg, int dense_t) {
-  for (int w = 0; w < MAXW; ++w) {
-    int segs = 0;
-    for (int g = 0; g < 4; ++g) segs += ((cfg.val_bits[w] >> (16 * g)) & 0xFFFFull) ? 1 : 0;
-    cfg.dense_tw[w] = dense_t > 64 || segs == 0 ? dense_t : std::max(1, (dense_t * segs + 3) / 4);
-  }
-}
-
 u32 narrow_mask(const Cfg& cfg, int W) {
   // Words whose SSSP slots all sit in lanes 0..15 can relax four edges per
   // lane-per-query step (phase N).  Measured on RMAT-22: worth it when every
@@ -2031,7 +2017,6 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       }
     }
     cfg.narrow_words = rshift >= 4 && cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, W) : 0u;
-    set_dense_thresholds(cfg, cfg.dense_t);
     // Bottom-up steps pay off only when few BFS queries share the words: with
     // many, the missing bits of a vertex are rarely all found early.  Hybrid
     // mode pulls only for batches of at most pull_max_queries BFS queries

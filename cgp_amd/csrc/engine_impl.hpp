@@ -264,9 +264,6 @@ void dispatch_seed(cgp_engine* en, int W, const KP& P, const int32_t* d_slot_src
 void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus);
 // Narrow-word mask of a slot configuration (all SSSP words narrow, else 0).
 u32 narrow_mask(const Cfg& cfg, int W);
-// Per-word lane-per-query thresholds of a slot configuration (scaled by the
-// 64-byte row segments that hold the word's SSSP slots).
-void set_dense_thresholds(Cfg& cfg, int dense_t);
 // One batched run (all passes, WCC, output transpose); stats may be null.
 void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out, bool out_dev,
                       cgp_run_stats* stats);

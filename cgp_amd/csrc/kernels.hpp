@@ -55,8 +55,7 @@ struct Cfg {
   int32_t rshift;       // log2(R)
   int32_t W;            // words in use
   int32_t pull_policy;  // 0 push only, 1 BFS pull whenever BFS bits are active, 2 thresholds
-  int32_t dense_t;      // dense_threshold option (full-word threshold)
-  int32_t dense_tw[MAXW];  // per word: popcount at which a vertex's SSSP pairs go lane-per-query
+  int32_t dense_t;      // popcount at which an SSSP word goes lane-per-query
   u32 delta;            // near-far step: thr moves to far_min + delta when no pair is below
                         // it (0: no postponing)
   u32 narrow_words;     // bit w: every SSSP slot of word w is in lanes 0..15 (and R >= 16), so

@@ -211,7 +211,6 @@ void admit(cgp_engine* en, ServeState* ss) {
   }
   if (!any) return;
   cfg.narrow_words = cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, ss->W) : 0u;
-  set_dense_thresholds(cfg, cfg.dense_t);
   HIP_CHECK(hipMemcpyAsync(en->d_cfg, en->h_cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipMemcpyAsync(ss->d_src.p, h_src, (size_t)ss->nslots * 4, hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipEventRecord(ss->staged, en->s));
@@ -397,7 +396,6 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     cfg.W = W;
     cfg.pull_policy = 0;
     cfg.dense_t = en->opts.dense_threshold;
-    set_dense_thresholds(cfg, cfg.dense_t);
     cfg.delta = (u32)en->opts.sssp_delta;
     cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
     Ctl& c = *en->h_ctl;
