@@ -746,69 +746,90 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
       __builtin_amdgcn_wave_barrier();
       const int cnt = (int)min(64u, total - win);
       // Per step: the lane's target value (offset or address), the weight
-      // and the two loaded values.
+      // and the two loaded values.  A window whose edges all belong to one
+      // item (chunks of high-degree vertices) reads that item's source row
+      // once instead of once per step.
       typedef typename std::conditional<A64, u64, u32>::type addr_t;
-      addr_t ot[U];
-      u32 wt[U], du[U], cur[U];
-      auto issue = [&](int k0) {
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-          const uint4 e = tab[k0 + k];  // k0 + k < 64: k0 < cnt <= 64 and U divides 64
+      const int jl = __builtin_amdgcn_readlane(j, cnt - 1);
+      const bool one_item = __builtin_amdgcn_readlane(j, 0) == jl;
+      u32 mine_lo = 0, mine_hi = 0;
+      auto steps = [&](auto one_c) {
+        constexpr bool ONE = decltype(one_c)::value;
+        u32 dsrc = 0;
+        if constexpr (ONE) {
           if constexpr (A64) {
-            ot[k] = (((u64)e.y << 32) | e.x) + off;
-            wt[k] = wtab[k0 + k];
-            du[k] = *(const gu32*)((((u64)e.w << 32) | e.z) + off);
-            cur[k] = *(const gu32*)ot[k];
+            const u64 sa1 = ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(srca >> 32), jl) << 32) |
+                            (u32)__builtin_amdgcn_readlane((int)(u32)srca, jl);
+            dsrc = *(const gu32*)(sa1 + off);
           } else {
-            ot[k] = e.x + off;
-            wt[k] = e.z;
-            du[k] = (u32)cgp_buf_load_i32(rs, (int)(e.y + off), 0, 0);
-            cur[k] = (u32)cgp_buf_load_i32(rs, (int)ot[k], 0, 0);
+            dsrc = (u32)cgp_buf_load_i32(rs, (int)((u32)__builtin_amdgcn_readlane((int)srcb, jl) + off), 0, 0);
+          }
+        }
+        addr_t ot[U];
+        u32 wt[U], du[U], cur[U];
+        auto issue = [&](int k0) {
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            const uint4 e = tab[k0 + k];  // k0 + k < 64: k0 < cnt <= 64 and U divides 64
+            if constexpr (A64) {
+              ot[k] = (((u64)e.y << 32) | e.x) + off;
+              wt[k] = wtab[k0 + k];
+              if constexpr (!ONE) du[k] = *(const gu32*)((((u64)e.w << 32) | e.z) + off);
+              cur[k] = *(const gu32*)ot[k];
+            } else {
+              ot[k] = e.x + off;
+              wt[k] = e.z;
+              if constexpr (!ONE) du[k] = (u32)cgp_buf_load_i32(rs, (int)(e.y + off), 0, 0);
+              cur[k] = (u32)cgp_buf_load_i32(rs, (int)ot[k], 0, 0);
+            }
+          }
+        };
+        issue(0);
+        for (int k0 = 0; k0 < cnt; k0 += U) {
+          u64 mk[U];
+          u32 c_c[U];
+          addr_t ot_c[U];
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            // Saturating add: a path past CGP_INF stays unreachable.
+            const u32 c = __builtin_elementwise_add_sat(ONE ? dsrc : du[k], wt[k]);
+            mk[k] = __ballot(c < cur[k]) & am;
+            c_c[k] = c;
+            ot_c[k] = ot[k];
+          }
+          if (k0 + U < cnt) issue(k0 + U);
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            const u64 m = mk[k];
+            if (m == 0) continue;
+            imp_n += (u32)__popcll(m);
+            // The improved lanes are SSSP lanes of the word, whose offset is
+            // their own value's.
+            if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+              if constexpr (A64)
+                __hip_atomic_fetch_min((gu32*)ot_c[k], c_c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              else
+                cgp_buf_atomic_umin_i32((int)c_c[k], rs, (int)ot_c[k], 0, 0);
+            }
+            if (!SPARSE) {
+              mine_lo = (u32)cgp_writelane_i32((int)(u32)m, k0 + k, (int)mine_lo);
+              mine_hi = (u32)cgp_writelane_i32((int)(u32)(m >> 32), k0 + k, (int)mine_hi);
+            } else {
+              const int32_t vk = __builtin_amdgcn_readlane(myv, k0 + k);
+              bool claimed = false;
+              if (lane == 0) {
+                ++t.nnew;
+                if (atomicOr(&P.next[nxt_at<W>(vk, wp)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
+              }
+              wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
+            }
           }
         }
       };
-      issue(0);
-      u32 mine_lo = 0, mine_hi = 0;
-      for (int k0 = 0; k0 < cnt; k0 += U) {
-        u64 mk[U];
-        u32 c_c[U];
-        addr_t ot_c[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-          // Saturating add: a path past CGP_INF stays unreachable.
-          const u32 c = __builtin_elementwise_add_sat(du[k], wt[k]);
-          mk[k] = __ballot(c < cur[k]) & am;
-          c_c[k] = c;
-          ot_c[k] = ot[k];
-        }
-        if (k0 + U < cnt) issue(k0 + U);
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-          const u64 m = mk[k];
-          if (m == 0) continue;
-          imp_n += (u32)__popcll(m);
-          // The improved lanes are SSSP lanes of the word, whose offset is
-          // their own value's.
-          if (__builtin_amdgcn_inverse_ballot_w64(m)) {
-            if constexpr (A64)
-              __hip_atomic_fetch_min((gu32*)ot_c[k], c_c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else
-              cgp_buf_atomic_umin_i32((int)c_c[k], rs, (int)ot_c[k], 0, 0);
-          }
-          if (!SPARSE) {
-            mine_lo = (u32)cgp_writelane_i32((int)(u32)m, k0 + k, (int)mine_lo);
-            mine_hi = (u32)cgp_writelane_i32((int)(u32)(m >> 32), k0 + k, (int)mine_hi);
-          } else {
-            const int32_t vk = __builtin_amdgcn_readlane(myv, k0 + k);
-            bool claimed = false;
-            if (lane == 0) {
-              ++t.nnew;
-              if (atomicOr(&P.next[nxt_at<W>(vk, wp)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
-            }
-            wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
-          }
-        }
-      }
+      // (Not in the 64-bit form: the second copy of the loop raises its
+      // registers past two waves per SIMD.)
+      if (!A64 && one_item) steps(std::true_type{});
+      else steps(std::false_type{});
       // Dense mode: lane f owns edge f's next-word update.
       if (!SPARSE && (mine_lo | mine_hi)) {
         ++t.nnew;
