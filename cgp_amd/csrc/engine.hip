@@ -1315,6 +1315,35 @@ __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, i
 }
 
 // ------------------------------------------------------------------ filter
+// Vertex i of the filter's list (i < n: the scan's i, or touched[i]), its next
+// words and CSR offsets; zeros past the end.  The W next words of a record
+// are contiguous and 16-byte aligned (W >= 2).
+template <int W>
+__device__ __forceinline__ void filter_fetch(const KP& P, u32 i, u32 n, bool scan, int32_t& v, u64 (&nw)[W],
+                                             int32_t& o0, int32_t& o1) {
+  const bool ok = i < n;
+  v = ok ? (scan ? (int32_t)i : P.touched[i]) : 0;
+  if (ok) {
+    if constexpr (W >= 2) {
+      const ulonglong2* q = (const ulonglong2*)&P.next[nxt_at<W>(v, 0)];
+#pragma unroll
+      for (int h = 0; h < W / 2; ++h) {
+        const ulonglong2 x = q[h];
+        nw[2 * h] = x.x;
+        nw[2 * h + 1] = x.y;
+      }
+    } else {
+      nw[0] = P.next[nxt_at<W>(v, 0)];
+    }
+    o0 = P.off[v];
+    o1 = P.off[v + 1];
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w) nw[w] = 0ull;
+    o0 = o1 = 0;
+  }
+}
+
 // Claims the next words of every touched vertex (sparse mode: the touched
 // list; dense mode: every vertex), settles BFS bits (visited |= new, depth =
 // iteration+1 - slot start, written as coalesced rows), publishes the
@@ -1343,105 +1372,178 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   }
   const u32 depth_base = iter + 1u;
   u64 pairs = 0, epairs = 0, fedges = 0;
-  for (u32 base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-    const u32 i = base + threadIdx.x;
-    const bool valid = i < n;
-    const int32_t v = valid ? (scan ? (int32_t)i : P.touched[i]) : 0;
-    u64 nwv[W];
-    bool anyw = false;
+  // Each thread takes FK vertices per step (their loads issued together) and
+  // the workgroup reserves their bin entries with one atomic per bin: the
+  // bin counters are single addresses every workgroup contends for (one
+  // reservation per 256 vertices cost about 0.3 ms per dense iteration on
+  // RMAT-22).
+  constexpr int FK = 4;
+  const u32 gstride = gridDim.x * BLOCK * FK;
+  // The next iteration's bin lists, picked once (indexing the argument's
+  // arrays with np inside the loop made the compiler copy the whole
+  // argument block to scratch memory).
+  int32_t* const l_small = np ? P.small[1] : P.small[0];
+  int32_t* const l_medium = np ? P.medium[1] : P.medium[0];
+  u64* const l_large = np ? P.large[1] : P.large[0];
+  u64* const l_dense = np ? P.dense[1] : P.dense[0];
+  u64* const l_dense_n = np ? P.dense_n[1] : P.dense_n[0];
+  for (u32 base = blockIdx.x * BLOCK * FK; base < n; base += gstride) {
+    int32_t vv[FK], o0[FK], o1[FK];
+    u64 nwk[FK][W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      nwv[w] = valid ? P.next[nxt_at<W>(v, w)] : 0ull;
-      anyw |= nwv[w] != 0;
-    }
-    const int32_t deg = anyw ? P.off[v + 1] - P.off[v] : 0;
-    u64 nb[W];
-    bool need_bin = false;
-    u32 dmask = 0, nmask = 0, vpairs = 0;  // dense words (wide / narrow)
+    for (int k = 0; k < FK; ++k)
+      filter_fetch<W>(P, base + (u32)k * BLOCK + threadIdx.x, n, scan, vv[k], nwk[k], o0[k], o1[k]);
+    u32 cnt[5] = {0u, 0u, 0u, 0u, 0u};
+    u32 kdeg[FK], kfl[FK];  // per vertex: degree; need_bin | bin << 1 | dense words << 4 | narrow << 12
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const u64 nw = nwv[w];
-      if (anyw) {
-        if (nw) P.next[nxt_at<W>(v, w)] = 0;
-        P.front[(size_t)v * W + w] = nw;
+    for (int k = 0; k < FK; ++k) {
+      const int32_t v = vv[k];
+      bool anyw = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) anyw |= nwk[k][w] != 0;
+      const int32_t deg = anyw ? o1[k] - o0[k] : 0;
+      u64 nb[W];
+      bool need_bin = false;
+      u32 dmask = 0, nmask = 0, vpairs = 0;  // dense words (wide / narrow)
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const u64 nw = nwk[k][w];
+        if (anyw) {
+          if (nw) P.next[nxt_at<W>(v, w)] = 0;
+          P.front[(size_t)v * W + w] = nw;
+        }
+        nb[w] = nw & bb[w];
+        if (nb[w]) P.visited[vis_at<W>(v, w)] |= nb[w];
+        const u64 s = nw & vb[w];
+        if (s && __popcll(s) >= dense_t) (((cfg->narrow_words >> w) & 1u) ? nmask : dmask) |= 1u << w;
+        else if (s) need_bin = true;
+        if (nb[w]) need_bin = true;
+        un[w] |= nw;
+        vpairs += (u32)__popcll(nb[w]);
       }
-      nb[w] = nw & bb[w];
-      if (nb[w]) P.visited[vis_at<W>(v, w)] |= nb[w];
-      const u64 s = nw & vb[w];
-      if (s && __popcll(s) >= dense_t) (((cfg->narrow_words >> w) & 1u) ? nmask : dmask) |= 1u << w;
-      else if (s) need_bin = true;
-      if (nb[w]) need_bin = true;
-      un[w] |= nw;
-      vpairs += (u32)__popcll(nb[w]);
-    }
-    pairs += vpairs;
-    epairs += (u64)vpairs * (u64)deg;
-    fedges += (u64)deg;
-    if (deg == 0) {
-      need_bin = false;
-      dmask = 0;
-      nmask = 0;
-    }
-    const int b = deg <= P.bin_small ? 0 : (deg <= P.bin_large ? 1 : 2);
-    {
-      // One reservation per workgroup and bin (4 global atomics per 256
-      // vertices instead of 4 per wave: the bin counters are single
-      // addresses every workgroup contends for).
+      pairs += vpairs;
+      epairs += (u64)vpairs * (u64)deg;
+      fedges += (u64)deg;
+      if (deg == 0) {
+        need_bin = false;
+        dmask = 0;
+        nmask = 0;
+      }
+      const u32 b = deg <= P.bin_small ? 0u : (deg <= P.bin_large ? 1u : 2u);
       const u32 dch = (u32)((deg + DCHUNK - 1) / DCHUNK);
-      u32 cnt[5], at[5];
-      cnt[0] = (need_bin && b == 0) ? 1u : 0u;
-      cnt[1] = (need_bin && b == 1) ? 1u : 0u;
-      cnt[2] = (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
-      cnt[3] = (u32)__popc(dmask) * dch;
-      cnt[4] = (u32)__popc(nmask) * dch;
-      block_reserve<5>(cnt, C->bcount[np], at, s_res);
-      if (cnt[0]) P.small[np][at[0]] = v;
-      if (cnt[1]) P.medium[np][at[1]] = v;
-      for (u32 c = 0; c < cnt[2]; ++c) P.large[np][at[2] + c] = ((u64)(u32)v << 32) | c;
-      u32 k = 0, kn = 0;
-      for (int w = 0; w < W; ++w) {
-        const bool wide = (dmask >> w) & 1u, narrow = (nmask >> w) & 1u;
-        for (u32 c = 0; c < dch && (wide || narrow); ++c) {
-          const u64 it = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
-          if (wide) P.dense[np][at[3] + k++] = it;
-          else P.dense_n[np][at[4] + kn++] = it;
+      cnt[0] += (need_bin && b == 0) ? 1u : 0u;
+      cnt[1] += (need_bin && b == 1) ? 1u : 0u;
+      cnt[2] += (need_bin && b == 2) ? (u32)((deg + LCHUNK - 1) / LCHUNK) : 0u;
+      cnt[3] += (u32)__popc(dmask) * dch;
+      cnt[4] += (u32)__popc(nmask) * dch;
+      kdeg[k] = (u32)deg;
+      kfl[k] = (need_bin ? 1u : 0u) | (b << 1) | (dmask << 4) | (nmask << 12);
+      // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
+      if (rshift == 6) {
+        // 64-slot rows: one row store per vertex with new bits, its vertex
+        // and new-bit mask read into scalar registers and the mask used as
+        // the store's exec mask.
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          u64 any = __ballot(nb[w] != 0);
+          if (!any) continue;
+          const u32 dval = depth_base - (u32)cfg->start[w * 64 + lane];
+          const u32 nlo = (u32)nb[w], nhi = (u32)(nb[w] >> 32);
+          while (any) {
+            const int j = __builtin_ctzll(any);
+            any &= any - 1;
+            const u64 nbj = ((u64)(u32)__builtin_amdgcn_readlane((int)nhi, j) << 32) |
+                            (u32)__builtin_amdgcn_readlane((int)nlo, j);
+            const int32_t vj = __builtin_amdgcn_readlane(v, j);
+            gu32* row = (gu32*)(P.vals + vrow(P.V, w, vj, 6));
+            if (__builtin_amdgcn_inverse_ballot_w64(nbj)) row[lane] = dval;
+          }
+        }
+      } else {
+        const int per = 64 >> rshift;
+        const int sub = lane >> rshift;
+        const int q = lane & ((1 << rshift) - 1);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const u64 any = __ballot(nb[w] != 0);
+          if (!any) continue;
+          const u32 dval = depth_base - (u32)cfg->start[w * 64 + q];
+          for (int j0 = 0; j0 < 64; j0 += per) {
+            const u64 grp = per == 64 ? ~0ull : (((1ull << per) - 1) << j0);
+            if (!(any & grp)) continue;
+            const int j = j0 + sub;
+            const u64 nbj = shfl64(nb[w], j);
+            const int32_t vj = __shfl(v, j, 64);
+            if ((nbj >> q) & 1ull) P.vals[vrow(P.V, w, vj, rshift) + q] = dval;
+          }
         }
       }
     }
-    // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
-    {
-      const int per = 64 >> rshift;
-      const int sub = lane >> rshift;
-      const int q = lane & ((1 << rshift) - 1);
+    u32 at[5];
+    block_reserve<5>(cnt, C->bcount[np], at, s_res);
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const u64 any = __ballot(nb[w] != 0);
-        if (!any) continue;
-        const u32 dval = depth_base - (u32)cfg->start[w * 64 + q];
-        for (int j0 = 0; j0 < 64; j0 += per) {
-          const u64 grp = per == 64 ? ~0ull : (((1ull << per) - 1) << j0);
-          if (!(any & grp)) continue;
-          const int j = j0 + sub;
-          const u64 nbj = shfl64(nb[w], j);
-          const int32_t vj = __shfl(v, j, 64);
-          if ((nbj >> q) & 1ull) P.vals[vrow(P.V, w, vj, rshift) + q] = dval;
+    for (int k = 0; k < FK; ++k) {
+      const int32_t v = vv[k];
+      const u32 deg = kdeg[k], fl = kfl[k];
+      const bool need_bin = fl & 1u;
+      const u32 b = (fl >> 1) & 3u, dmask = (fl >> 4) & 0xFFu, nmask = (fl >> 12) & 0xFFu;
+      if (need_bin && b == 0) l_small[at[0]++] = v;
+      if (need_bin && b == 1) l_medium[at[1]++] = v;
+      if (need_bin && b == 2) {
+        const u32 nch = (deg + LCHUNK - 1) / LCHUNK;
+        for (u32 c = 0; c < nch; ++c) l_large[at[2]++] = ((u64)(u32)v << 32) | c;
+      }
+      if (dmask | nmask) {
+        const u32 dch = (deg + DCHUNK - 1) / DCHUNK;
+        for (int w = 0; w < W; ++w) {
+          const bool wide = (dmask >> w) & 1u, narrow = (nmask >> w) & 1u;
+          for (u32 c = 0; c < dch && (wide || narrow); ++c) {
+            const u64 it = ((u64)(u32)v << 32) | ((u64)w << 24) | c;
+            if (wide) l_dense[at[3]++] = it;
+            else l_dense_n[at[4]++] = it;
+          }
         }
       }
     }
   }
-  // Block-level reductions of the frontier statistics.
-  pairs = wave_sum64(pairs);
-  epairs = wave_sum64(epairs);
-  fedges = wave_sum64(fedges);
-  if (lane == 0) {
-    if (pairs) atomicAdd(&C->bfs_pairs[np], pairs);
-    if (epairs) atomicAdd(&C->bfs_edges[np], epairs);
-    if (fedges) atomicAdd(&C->fedges[np], fedges);
-  }
+  // Frontier statistics: wave sums, then one atomic per workgroup and counter.
+  {
+    __shared__ u64 s_st[(BLOCK / 64) * (3 + MAXW)];
+    const int wave = (int)(threadIdx.x >> 6);
+    pairs = wave_sum64(pairs);
+    epairs = wave_sum64(epairs);
+    fedges = wave_sum64(fedges);
+    u64 unw[W];
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    const u64 x = wave_or64(un[w]);
-    if (lane == 0 && x) atomicOr(&C->union_[w], x);
+    for (int w = 0; w < W; ++w) unw[w] = wave_or64(un[w]);
+    if (lane == 0) {
+      u64* r = s_st + wave * (3 + MAXW);
+      r[0] = pairs;
+      r[1] = epairs;
+      r[2] = fedges;
+#pragma unroll
+      for (int w = 0; w < W; ++w) r[3 + w] = unw[w];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u64 tp = 0, te = 0, tf = 0, tu[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) tu[w] = 0;
+      for (int wv = 0; wv < BLOCK / 64; ++wv) {
+        const u64* r = s_st + wv * (3 + MAXW);
+        tp += r[0];
+        te += r[1];
+        tf += r[2];
+#pragma unroll
+        for (int w = 0; w < W; ++w) tu[w] |= r[3 + w];
+      }
+      if (tp) atomicAdd(&C->bfs_pairs[np], tp);
+      if (te) atomicAdd(&C->bfs_edges[np], te);
+      if (tf) atomicAdd(&C->fedges[np], tf);
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (tu[w]) atomicOr(&C->union_[w], tu[w]);
+    }
   }
   // Last workgroup: advance the state machine.
   __syncthreads();
