@@ -1655,16 +1655,48 @@ __global__ void k_fill_u32(u32* p, size_t n, u32 val) {
 // rows: out[qmap[w*64+q] * V + v].  One workgroup per 64-vertex tile.
 // Transposes OUT_VT vertices x R slots of one word's value rows into the
 // query-major output (each output row segment is OUT_VT * 4 = 256 bytes;
-// 128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22).
+// 128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22; 16-byte global
+// accesses took 2.9 to 1.55 ms).
 constexpr int OUT_VT = 64;
 __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, int32_t V, int rshift,
-                                                  const int32_t* qmap, u32* out) {
+                                                  const int32_t* qmap, u32* out, int vec) {
   __shared__ u32 tile[64][OUT_VT + 1];
   const int w = blockIdx.y;
   const int R = 1 << rshift;
   const int32_t v0 = (int32_t)blockIdx.x * OUT_VT;
   const int nv = min(OUT_VT, V - v0);
   const size_t rowbase = vrow(V, w, v0, rshift);
+  if (vec) {
+    // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by the
+    // launcher): 16-byte loads of 4 slots of a vertex, 16-byte stores of 4
+    // vertices of a query.
+    const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int j = pass * 16 + sub;
+      if (j < nv) {
+        const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
+        tile[c4 + 0][j] = x.x;
+        tile[c4 + 1][j] = x.y;
+        tile[c4 + 2][j] = x.z;
+        tile[c4 + 3][j] = x.w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int q = pass * 16 + sub;
+      const int32_t dst = qmap[w * 64 + q];
+      if (dst < 0) continue;
+      u32* o = out + (size_t)dst * V + v0 + c4;
+      if (c4 + 4 <= nv) {
+        *(uint4*)o = make_uint4(tile[q][c4], tile[q][c4 + 1], tile[q][c4 + 2], tile[q][c4 + 3]);
+      } else {
+        for (int i = 0; c4 + i < nv; ++i) o[i] = tile[q][c4 + i];
+      }
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
     const int j = t >> rshift, q = t & (R - 1);
     if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
@@ -2215,8 +2247,9 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->stamp_epoch += hc.iter + 2;
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
     en->tick(CAT_OUT, true);
+    const int vec = rshift == 6 && (V & 3) == 0 && ((uintptr_t)dout & 15) == 0;
     hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK), 0, en->s,
-                       (const u32*)en->vals.p, V, rshift, (const int32_t*)en->qmap.p, dout);
+                       (const u32*)en->vals.p, V, rshift, (const int32_t*)en->qmap.p, dout, vec);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
   }
