@@ -254,8 +254,9 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--pmc-dir", default="",
                     help="directory with rocprofv3 FETCH_SIZE/WRITE_SIZE CSVs of this workload: fills roofline.traffic")
-    ap.add_argument("--pmc-runs", type=int, default=3,
-                    help="batch runs in each profiled command (bench.py --steps 1 --warmup 0 makes 3)")
+    ap.add_argument("--pmc-runs", type=int, default=2 + COUNTED_RUNS,
+                    help="batch runs in each profiled command (bench.py --steps 1 --warmup 0 makes "
+                         "one capture run, one timed step and the counted runs: %d)" % (2 + COUNTED_RUNS))
     ap.add_argument("--no-one-at-a-time", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
