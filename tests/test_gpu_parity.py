@@ -175,6 +175,33 @@ def test_lane_per_query_wide_addresses(gpu, unroll, dense_t, div):
     assert np.array_equal(out, oracle_rows(og, kinds, sources))
 
 
+@pytest.mark.parametrize("layout", ["v_odd", "aligned", "offset4"])
+def test_output_transpose_layouts(gpu, layout):
+    # Query-major results through both output transposes with 64-slot rows:
+    # 16-byte accesses (V % 4 == 0, 16-byte aligned output) and the
+    # per-element path (V % 4 != 0, or an output 4 bytes past alignment).
+    say("output layout", layout)
+    if layout == "v_odd":
+        h = cgp_amd.HostCSR.grid(31, 33, 0.05, seed=3, wmin=1, wmax=50)  # V = 1023
+    else:
+        h = cgp_amd.HostCSR.rmat(10, 8, seed=4, wmin=1, wmax=50)  # V = 1024
+    g, og = cgp_amd.Graph.from_host(h), O.G.of(h)
+    src = [int(x) for x in h.sample_sources(100, seed=7)]
+    kinds = [(BFS, SSSP)[i % 2] for i in range(100)] + [WCC]
+    sources = src + [0]
+    exp = oracle_rows(og, kinds, sources)
+    en = cgp_amd.Engine(g)
+    n, V = len(kinds), g.V
+    if layout == "v_odd":
+        out, _ = en.run_batch(kinds, sources)
+    else:
+        off = 0 if layout == "aligned" else 4
+        buf = cgp_amd.DeviceBuffer(n * V * 4 + 16)
+        en.run_batch(kinds, sources, out=buf.view(off, n * V * 4))
+        out = buf.to_host()[off // 4: off // 4 + n * V].reshape(n, V)
+    assert np.array_equal(out, exp)
+
+
 @pytest.mark.parametrize("div", [1, 1000000])
 @pytest.mark.parametrize("n_sssp,n_bfs", [(16, 0), (12, 100), (16, 48)])
 def test_narrow_lane_per_query_words(gpu, div, n_sssp, n_bfs):
