@@ -1,0 +1,57 @@
+"""bench.py's host-side contract: the one-GPU guard and the HBM-traffic
+calibration of the roofline line (no GPU needed)."""
+import csv
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def write_counters(path, counter, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        for name, kib in rows:
+            w.writerow({"Kernel_Name": name, "Counter_Name": counter, "Counter_Value": kib})
+
+
+def test_pmc_traffic_calibrates_on_the_output_kernel(tmp_path):
+    # k_output reads words*V*R*4 bytes and writes out_rows*V*4 bytes.  The
+    # synthetic counters report half of its reads (FETCH factor 2) and all of
+    # its writes (WRITE factor 1), so the advance kernel's calibrated traffic
+    # is 2 x its FETCH plus its WRITE, divided by runs x working launches.
+    V, words, R, out_rows = 1024, 2, 64, 100
+    fetch_out = 4.0 * words * V * R / 2 / 1024  # KiB as the counter reports it
+    write_out = 4.0 * out_rows * V / 1024
+    adv = "void (anonymous namespace)::k_adv_dense<2, 256, false, 16, 1>(cgpk::KP)"
+    other = "void (anonymous namespace)::k_adv_push<2, 256>(cgpk::KP)"
+    (tmp_path / "fetch").mkdir()
+    (tmp_path / "write").mkdir()
+    write_counters(tmp_path / "fetch" / "f_counter_collection.csv", "FETCH_SIZE",
+                   [("k_output(unsigned int const*, int)", fetch_out), (adv, 100.0), (adv, 300.0),
+                    (other, 5000.0), ("k_output(unsigned int const*, int)", fetch_out)])
+    write_counters(tmp_path / "write" / "w_counter_collection.csv", "WRITE_SIZE",
+                   [("k_output(unsigned int const*, int)", write_out), (adv, 40.0), (other, 7000.0)])
+    traffic, factors = bench.pmc_traffic(str(tmp_path), "k_adv_dense<", out_rows, V, words, R,
+                                         runs=2, launches_per_run=1.0)
+    assert factors["fetch_size_factor"] == pytest.approx(2.0)
+    assert factors["write_size_factor"] == pytest.approx(1.0)
+    assert traffic == pytest.approx((2.0 * (100.0 + 300.0) + 40.0) * 1024.0 / 2.0)
+
+
+def test_pmc_traffic_without_calibration_rows_is_none(tmp_path):
+    write_counters(tmp_path / "x_counter_collection.csv", "FETCH_SIZE",
+                   [("void k_adv_dense<1, 256, false, 16, 1>(KP)", 10.0)])
+    assert bench.pmc_traffic(str(tmp_path), "k_adv_dense<", 1, 64, 1, 64, 1, 1.0) == (None, None)
