@@ -712,7 +712,9 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
   for (int wp = 0; wp < W; ++wp) {
     const u64 vb = P.cfg->val_bits[wp];
     if (vb == 0 || __ballot(dlen != 0 && w == wp) == 0) continue;
-    const i32x4 rs = raw_rsrc((u64)P.vals + (u64)wp * wordb, wordb);
+    // 32-bit form: a descriptor over this word's rows (unused, and not
+    // formed, in the 64-bit form, where wordb may not fit 32 bits).
+    const i32x4 rs = A64 ? i32x4{0, 0, 0, 0} : raw_rsrc((u64)P.vals + (u64)wp * wordb, wordb);
     // This word's SSSP lanes read their own value; the others read the
     // row's first value (the same lines) and are masked out.
     const bool inw = inrow && ((vb >> lq) & 1ull) != 0;
