@@ -2433,7 +2433,7 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->bin_small = 64;
   o->bin_large = 2048;
   o->use_hip_graph = 1;
-  o->dense_threshold = 8;
+  o->dense_threshold = 0;  // automatic (resolved at engine creation)
   o->frontier_dense_div = 8;
   o->dense_unroll = 16;
   o->dense_narrow = 1;
@@ -2472,7 +2472,7 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     if (en->opts.bin_large < en->opts.bin_small) en->opts.bin_large = en->opts.bin_small;
     if (en->opts.block_size != 128 && en->opts.block_size != 256 && en->opts.block_size != 512)
       cgpi::fail(CGP_EINVAL, "cgp_engine_create: block_size must be 128, 256 or 512");
-    if (en->opts.dense_threshold <= 0) en->opts.dense_threshold = d.dense_threshold;
+    if (en->opts.dense_threshold < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_threshold must be >= 0");
     if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
     if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
     if (en->opts.advance_blocks < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: advance_blocks must be >= 0");
@@ -2491,6 +2491,14 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
       const double deg = g->V > 0 ? (double)g->E / (double)g->V : 0.0;
       en->opts.sssp_delta =
           (g->weighted && deg <= 8.0 && g->mean_weight > 1.0) ? (int32_t)std::lround(20.0 * g->mean_weight) : 0;
+    }
+    if (en->opts.dense_threshold == 0) {
+      // Automatic lane-per-query threshold.  The table loop (postponing off)
+      // costs less per edge row than the postponing loop, so it pays from
+      // fewer active queries: 6 took the RMAT-22 mixed batch from 68.3 to
+      // 66.5 ms, while on the 4096^2 grid (postponing on) 6 instead of 8
+      // took 2380 to 2535 ms.
+      en->opts.dense_threshold = en->opts.sssp_delta == 0 ? 6 : 8;
     }
     en->cus = device_cus();
     {

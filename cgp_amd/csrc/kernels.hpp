@@ -43,7 +43,11 @@ constexpr int MAXW = 8;          // mask words per pass (512 slots)
 constexpr u32 INF = 0xFFFFFFFFu;
 constexpr int BLOCK = 256;       // workgroup size of every kernel (4 waves)
 constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
-constexpr int DCHUNK = 512;      // edges per dense lane-per-query wave item
+// Edges per dense lane-per-query wave item.  A wave claims 64 items at a
+// time, so the chunk bounds the work of the last claims of a launch: on
+// RMAT-22 (80 SSSP) 1024 / 512 / 256 / 128 / 64 edges measured 37.2 / 32.1 /
+// 29.6 / 28.8 / 30.0 ms of lane-per-query time per batch.
+constexpr int DCHUNK = 128;
 constexpr int NPROF = 15;
 constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
 

@@ -84,7 +84,8 @@ typedef struct {
                                   bit 1: device work counters -> advance_bytes */
   uint64_t hbm_budget_bytes;   /* serving: cap on per-query value memory; 0 = 64 GiB */
   int32_t dense_threshold;     /* active SSSP queries in a mask word at which the word is relaxed
-                                  lane-per-query (one wave per edge row); 0 = 8 */
+                                  lane-per-query (one wave per edge row); 0 = automatic: 6 when
+                                  sssp_delta resolves to 0 (postponing off), else 8 */
   int32_t frontier_dense_div;  /* dense frontier mode (fire-and-forget atomics, filter scans all
                                   vertices) once the frontier's edges reach V / div; 0 = 8 */
   int32_t dense_unroll;        /* value rows in flight per step of the lane-per-query kernel: 4, 8

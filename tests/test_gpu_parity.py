@@ -246,12 +246,16 @@ def test_near_far_grid_long_paths(gpu):
     en = cgp_amd.Engine(g)
     w = np.asarray(h.weights, dtype=np.float64)
     assert en.opts["sssp_delta"] == int(round(20 * w.mean()))
+    assert en.opts["dense_threshold"] == 8  # automatic threshold with postponing on
     assert np.array_equal(en.run_batch([SSSP] * 4, src)[0], ref)
 
 
 def test_near_far_automatic_off_on_small_world(gpu):
     h, g, og = rmat(12, 16, 4, 1, 255)
-    assert cgp_amd.Engine(g).opts["sssp_delta"] == 0
+    en = cgp_amd.Engine(g)
+    assert en.opts["sssp_delta"] == 0
+    assert en.opts["dense_threshold"] == 6  # automatic threshold with postponing off
+    assert cgp_amd.Engine(g, dense_threshold=3).opts["dense_threshold"] == 3
 
 
 def test_alone_equals_in_the_mix(gpu):
