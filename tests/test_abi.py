@@ -81,3 +81,45 @@ def test_cpp_facade_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", src], capture_output=True,
                        text=True)
     assert r.returncode == 0, r.stderr
+
+
+STRUCTS = ["cgp_query", "cgp_engine_opts", "cgp_run_stats", "cgp_iteration_record", "cgp_serve_opts",
+           "cgp_completion"]
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    # Size and every field offset of the ctypes mirrors equal what a C
+    # compiler lays out from include/cgp/cgp.h.
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "cgp/cgp.h"', "int main(void) {"]
+    for s in STRUCTS:
+        py = getattr(_lib, s)
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (s, s))
+        for name, _ in py._fields_:
+            cname = "pass" if name == "pass_" else name
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (s, name, s, cname))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=120)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        s, name, val = line.split()
+        got[(s, name)] = int(val)
+    for s in STRUCTS:
+        py = getattr(_lib, s)
+        assert got[(s, "size")] == C.sizeof(py), s
+        for name, _ in py._fields_:
+            assert got[(s, name)] == getattr(py, name).offset, (s, name)
+
+
+def test_engine_option_defaults():
+    o = _lib.cgp_engine_opts()
+    _lib.lib.cgp_engine_opts_default(C.byref(o))
+    assert o.dense_threshold == 0  # automatic: resolved from sssp_delta at engine creation
+    assert o.dense_unroll == 16
+    assert o.sssp_delta == -1
+    assert o.dense_wide_addr == 0
+    assert o.concurrent_advance == 0
+    assert o.block_size == 256
