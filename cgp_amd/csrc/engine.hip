@@ -1857,6 +1857,19 @@ void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, int tab) {
   }
 }
 
+// Optional advance launches of the current slot configuration: bit 0, the
+// lane-per-query kernels (some word has SSSP slots); bit 1, the narrow
+// kernel (some word is narrow).  The filter routes no item to a kernel that
+// is not launched, and a BFS-only batch saves three launches per iteration.
+int launch_set(const cgp_engine* en) {
+  const Cfg& c = *en->h_cfg;
+  int ls = 0;
+  for (int w = 0; w < MAXW; ++w)
+    if (c.val_bits[w]) ls |= 1;
+  if (c.narrow_words) ls |= 2;
+  return ls;
+}
+
 // Lane-per-query loop of an engine: the table loop when postponing is off,
 // with 32-bit byte offsets when one word's value rows (at the widest row,
 // 64 slots) span less than 4 GiB; the postponing loop otherwise.
@@ -1897,8 +1910,9 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   } else {
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, sp, P);
   }
-  launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_mode(en));
-  hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
+  const int ls = launch_set(en);
+  if (ls & 1) launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_mode(en));
+  if (ls & 2) hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (en->opts.concurrent_advance) {
     HIP_CHECK(hipEventRecord(en->ev_join, en->s2));
     HIP_CHECK(hipStreamWaitEvent(s, en->ev_join, 0));
@@ -2034,7 +2048,8 @@ static void run_iterations(cgp_engine* en, int W) {
   const bool use_graph = en->opts.use_hip_graph != 0;
   const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
   cgp_engine::GExec& gx = en->gx[wlog(W)];
-  if (use_graph && (!gx.exec || gx.K != K || gx.prof != en->opts.profile)) {
+  const int ls = launch_set(en);
+  if (use_graph && (!gx.exec || gx.K != K || gx.prof != en->opts.profile || gx.ls != ls)) {
     if (gx.exec) (void)hipGraphExecDestroy(gx.exec);
     gx.exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -2046,6 +2061,7 @@ static void run_iterations(cgp_engine* en, int W) {
     HIP_CHECK(ie);
     gx.K = K;
     gx.prof = en->opts.profile;
+    gx.ls = ls;
   }
   for (;;) {
     if (use_graph) {

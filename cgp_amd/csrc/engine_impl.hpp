@@ -123,6 +123,7 @@ struct cgp_engine {
     hipGraphExec_t exec = nullptr;
     int K = 0;
     int prof = -1;
+    int ls = -1;  // optional launches captured (launch_set)
   };
   GExec gx[4];
   double clock_khz = 100000.0;  // device wall-clock rate (wall_clock64 ticks)
