@@ -25,6 +25,11 @@
  *                    golden T1 vector.
  *   orc_bellman_ford closed-form small graphs (paths, stars, cycles).
  *   orc_reached_edges  closed forms (full graph reachability -> E).
+ *   orc_check_sssp / orc_check_wcc / orc_check_rows
+ *                    accept every oracle answer and reject planted mistakes
+ *                    (a depth or distance off by one either way, a reachable
+ *                    vertex left unreached, two components merged under one
+ *                    label, a label that is not the component minimum).
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -274,4 +279,176 @@ double orc_run_queries(int32_t V, const int32_t* off, const int32_t* idx, const 
   free(th);
   pthread_mutex_destroy(&pool.mu);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ------------------------------------------------------------------ result checkers
+ * Necessary-and-sufficient characterisations of the unique answers, so a
+ * result can be checked at sizes where running the oracle per query is too
+ * slow.  They read only the graph and the candidate result. */
+
+/* Error codes of the checkers (0 = the result is the exact answer). */
+enum {
+  ORC_OK = 0,
+  ORC_ESOURCE = 1,   /* d[src] != 0, or src out of range with a finite value */
+  ORC_ERELAX = 2,    /* an edge (u,v) with d[v] > d[u] + w: some vertex too far */
+  ORC_ETIGHT = 3,    /* a finite d[v] (v != src) with no in-edge (u,v), d[u] + w == d[v] */
+  ORC_EZERO = 4,     /* d == 0 away from the source */
+  ORC_EEDGE = 5,     /* WCC: an edge joins two different labels */
+  ORC_EROOT = 6,     /* WCC: label[v] > v, or label[label[v]] != label[v] */
+  ORC_ESPLIT = 7,    /* WCC: a label class is not connected */
+  ORC_EKIND = 8      /* unknown query kind */
+};
+
+/* d is the exact shortest-path distance vector from src (ORC_INF = unreachable)
+ * on a graph whose weights are all >= 1 (wt == NULL: unit weights, i.e. BFS
+ * depths) iff
+ *   (a) d[src] == 0 and no other vertex has d == 0;
+ *   (b) every edge (u,v) with finite d[u] has d[v] <= d[u] + w(u,v);
+ *   (c) every finite d[v], v != src, has a tight in-edge: d[u] + w(u,v) == d[v].
+ * (b) along a shortest path gives d <= dist; following tight in-edges back from
+ * v strictly decreases d (w >= 1) and can only stop at the one zero, src, so
+ * d[v] is the length of a real path and d >= dist; unreachable vertices can
+ * have no such chain, so they stay ORC_INF.  `tight` is caller scratch of V
+ * bytes. */
+static int check_sssp_row(int32_t V, const int32_t* off, const int32_t* idx, const uint32_t* wt,
+                          int32_t src, const uint32_t* d, unsigned char* tight) {
+  if (V == 0) return ORC_OK;
+  if (src < 0 || src >= V || d[src] != 0) return ORC_ESOURCE;
+  memset(tight, 0, (size_t)V);
+  for (int32_t u = 0; u < V; ++u) {
+    const uint32_t du = d[u];
+    if (du == ORC_INF) continue;
+    for (int32_t e = off[u]; e < off[u + 1]; ++e) {
+      const int32_t v = idx[e];
+      const uint64_t cand = (uint64_t)du + (wt ? wt[e] : 1u);
+      const uint64_t dv = d[v];
+      if (dv > cand) return ORC_ERELAX;
+      if (dv == cand) tight[v] = 1;
+    }
+  }
+  for (int32_t v = 0; v < V; ++v) {
+    if (v == src || d[v] == ORC_INF) continue;
+    if (d[v] == 0) return ORC_EZERO;
+    if (!tight[v]) return ORC_ETIGHT;
+  }
+  return ORC_OK;
+}
+
+int orc_check_sssp(int32_t V, const int32_t* off, const int32_t* idx, const uint32_t* wt, int32_t src,
+                   const uint32_t* d) {
+  unsigned char* tight = (unsigned char*)malloc((size_t)(V > 0 ? V : 1));
+  const int r = check_sssp_row(V, off, idx, wt, src, d, tight);
+  free(tight);
+  return r;
+}
+
+/* labels are the minimum-vertex-id labels of the weakly connected components
+ * iff
+ *   (a) label[u] == label[v] on every edge: each label class is a union of
+ *       components;
+ *   (b) label[v] <= v and label[label[v]] == label[v]: the label of a class is
+ *       a member of it and no member is smaller, so it is the class minimum;
+ *   (c) every class is connected: a traversal over the edges taken as
+ *       undirected, started at each class's own label vertex, reaches every
+ *       vertex exactly once in total (a class of two components would leave
+ *       the second unreached).
+ * Works on directed graphs too (weak components): (c) walks out- and in-edges. */
+int orc_check_wcc(int32_t V, const int32_t* off, const int32_t* idx, const uint32_t* lab) {
+  if (V == 0) return ORC_OK;
+  const int64_t E = off[V];
+  for (int32_t u = 0; u < V; ++u)
+    for (int32_t e = off[u]; e < off[u + 1]; ++e)
+      if (lab[u] != lab[idx[e]]) return ORC_EEDGE;
+  for (int32_t v = 0; v < V; ++v)
+    if (lab[v] > (uint32_t)v || lab[lab[v]] != lab[v]) return ORC_EROOT;
+  /* In-edges (reverse CSR by counting sort) for the undirected walk. */
+  int64_t* roff = (int64_t*)calloc((size_t)V + 1, sizeof(int64_t));
+  int32_t* rsrc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1));
+  int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)V);
+  unsigned char* seen = (unsigned char*)calloc((size_t)V, 1);
+  for (int64_t e = 0; e < E; ++e) ++roff[idx[e] + 1];
+  for (int32_t v = 0; v < V; ++v) roff[v + 1] += roff[v];
+  {
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)V);
+    memcpy(fill, roff, sizeof(int64_t) * (size_t)V);
+    for (int32_t u = 0; u < V; ++u)
+      for (int32_t e = off[u]; e < off[u + 1]; ++e) rsrc[fill[idx[e]]++] = u;
+    free(fill);
+  }
+  int64_t reached = 0;
+  for (int32_t r = 0; r < V; ++r) {
+    if (lab[r] != (uint32_t)r) continue;
+    int64_t top = 0;
+    stack[top++] = r;
+    seen[r] = 1;
+    while (top > 0) {
+      const int32_t u = stack[--top];
+      ++reached;
+      for (int32_t e = off[u]; e < off[u + 1]; ++e)
+        if (!seen[idx[e]]) { seen[idx[e]] = 1; stack[top++] = idx[e]; }
+      for (int64_t e = roff[u]; e < roff[u + 1]; ++e)
+        if (!seen[rsrc[e]]) { seen[rsrc[e]] = 1; stack[top++] = rsrc[e]; }
+    }
+  }
+  free(roff);
+  free(rsrc);
+  free(stack);
+  free(seen);
+  return reached == V ? ORC_OK : ORC_ESPLIT;
+}
+
+/* Checks n result rows (row i at rows + i * row_stride, kinds 0 BFS / 1 SSSP
+ * / 2 WCC, sources srcs[i]) on `threads` threads, one row per thread at a
+ * time; status[i] receives the row's error code.  WCC rows are checked with
+ * orc_check_wcc.  Returns the number of rows that failed. */
+typedef struct {
+  int32_t V;
+  const int32_t *off, *idx;
+  const uint32_t* wt;
+  const int32_t *kinds, *srcs;
+  int32_t n;
+  const uint32_t* rows;
+  int64_t stride;
+  int32_t* status;
+  volatile int32_t next;
+  pthread_mutex_t mu;
+} orc_check_pool;
+
+static void* orc_check_worker(void* arg) {
+  orc_check_pool* p = (orc_check_pool*)arg;
+  unsigned char* tight = (unsigned char*)malloc((size_t)(p->V > 0 ? p->V : 1));
+  for (;;) {
+    pthread_mutex_lock(&p->mu);
+    const int32_t i = p->next++;
+    pthread_mutex_unlock(&p->mu);
+    if (i >= p->n) break;
+    const uint32_t* row = p->rows + (size_t)i * (size_t)p->stride;
+    const int32_t k = p->kinds[i];
+    if (k == 0) p->status[i] = check_sssp_row(p->V, p->off, p->idx, NULL, p->srcs[i], row, tight);
+    else if (k == 1) p->status[i] = check_sssp_row(p->V, p->off, p->idx, p->wt, p->srcs[i], row, tight);
+    else if (k == 2) p->status[i] = orc_check_wcc(p->V, p->off, p->idx, row);
+    else p->status[i] = ORC_EKIND;
+  }
+  free(tight);
+  return NULL;
+}
+
+int32_t orc_check_rows(int32_t V, const int32_t* off, const int32_t* idx, const uint32_t* wt,
+                       const int32_t* kinds, const int32_t* srcs, int32_t n, const uint32_t* rows,
+                       int64_t row_stride, int32_t threads, int32_t* status) {
+  orc_check_pool pool;
+  pool.V = V; pool.off = off; pool.idx = idx; pool.wt = wt;
+  pool.kinds = kinds; pool.srcs = srcs; pool.n = n; pool.rows = rows;
+  pool.stride = row_stride; pool.status = status; pool.next = 0;
+  pthread_mutex_init(&pool.mu, NULL);
+  if (threads < 1) threads = 1;
+  if (threads > n) threads = n > 0 ? n : 1;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+  for (int32_t i = 0; i < threads; ++i) pthread_create(&th[i], NULL, orc_check_worker, &pool);
+  for (int32_t i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  pthread_mutex_destroy(&pool.mu);
+  int32_t bad = 0;
+  for (int32_t i = 0; i < n; ++i) bad += status[i] != ORC_OK;
+  return bad;
 }

@@ -58,11 +58,15 @@ def check_bfs(off, idx, source, d):
 
 
 def check_wcc(off, idx, labels):
-    """labels are min-id component labels iff: labels equal across every edge;
-    label[v] <= v; label[label[v]] == label[v] (the label is a member, its own
-    root); and each label class is connected is implied by the first two with
-    the root being the minimum (label[v] <= v and the class representative
-    labels itself)."""
+    """labels are the min-id labels of the weakly connected components iff:
+    (a) labels are equal across every edge, so each label class is a union of
+    components; (b) label[v] <= v and label[label[v]] == label[v], so a class's
+    label is a member no larger than any other member, its minimum; (c) there
+    are as many label classes as components (scipy's connected_components, an
+    independent count), so no class holds two components.  Without (c) a
+    labelling that put components {0} and {1, 2} under label 0 would pass."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
     lab = np.asarray(labels, dtype=np.int64)
     V = len(off) - 1
     src = np.repeat(np.arange(V, dtype=np.int64), np.diff(off.astype(np.int64)))
@@ -72,4 +76,8 @@ def check_wcc(off, idx, labels):
         return "label above vertex id"
     if np.any(lab[lab] != lab):
         return "label not a fixed point"
+    a = csr_matrix((np.ones(len(idx), dtype=np.int8), idx.astype(np.int64), off.astype(np.int64)), shape=(V, V))
+    ncomp, _ = connected_components(a, directed=True, connection="weak")
+    if np.count_nonzero(lab == np.arange(V)) != ncomp:
+        return "label class not connected"
     return None

@@ -28,6 +28,13 @@ def _load_oracle():
     lib.orc_run_queries.argtypes = [C.c_int32, _i32p, _i32p, _u32p, _i32p, _i32p, C.c_int32,
                                     C.c_int32, _u32p, _u64p]
     lib.orc_run_queries.restype = C.c_double
+    lib.orc_check_sssp.argtypes = [C.c_int32, _i32p, _i32p, _u32p, C.c_int32, _u32p]
+    lib.orc_check_sssp.restype = C.c_int
+    lib.orc_check_wcc.argtypes = [C.c_int32, _i32p, _i32p, _u32p]
+    lib.orc_check_wcc.restype = C.c_int
+    lib.orc_check_rows.argtypes = [C.c_int32, _i32p, _i32p, _u32p, _i32p, _i32p, C.c_int32, _u32p,
+                                   C.c_int64, C.c_int32, _i32p]
+    lib.orc_check_rows.restype = C.c_int32
     return lib
 
 
@@ -148,3 +155,45 @@ def ref_call(name, g, *args):
     else:
         fn(g.V, off, idx, int(args[0]), op)
     return o
+
+
+# Error codes of the C checkers (0 = exact answer).
+CHECK_ERRORS = {0: None, 1: "source not 0", 2: "edge not relaxed", 3: "finite vertex without tight in-edge",
+                4: "zero distance off the source", 5: "edge crosses labels", 6: "label not the class minimum",
+                7: "label class not connected", 8: "unknown kind"}
+
+
+def host_threads():
+    """Worker threads for the checkers: the host's cores, at most 16 (a GPU
+    box's CPU share)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def check_sssp(g, source, d, unit=False):
+    """Error text, or None when d is the exact distance (unit: depth) vector."""
+    v = np.ascontiguousarray(d, dtype=np.uint32)
+    off, idx, wt = g.p()
+    return CHECK_ERRORS[orc.orc_check_sssp(g.V, off, idx, None if unit else wt, int(source),
+                                            v.ctypes.data_as(_u32p))]
+
+
+def check_wcc(g, labels):
+    v = np.ascontiguousarray(labels, dtype=np.uint32)
+    off, idx, _ = g.p()
+    return CHECK_ERRORS[orc.orc_check_wcc(g.V, off, idx, v.ctypes.data_as(_u32p))]
+
+
+def check_rows(g, kinds, sources, rows, threads=None):
+    """Checks every row of a [n, V] uint32 result (kinds 0 BFS / 1 SSSP / 2 WCC)
+    on host threads; returns the list of (row, error text) that failed."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    n = len(kinds)
+    assert rows.shape == (n, g.V)
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    s = np.ascontiguousarray(sources, dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    off, idx, wt = g.p()
+    orc.orc_check_rows(g.V, off, idx, wt, k.ctypes.data_as(_i32p), s.ctypes.data_as(_i32p), n,
+                       rows.ctypes.data_as(_u32p), g.V, int(threads or host_threads()),
+                       st.ctypes.data_as(_i32p))
+    return [(i, CHECK_ERRORS[int(st[i])]) for i in range(n) if st[i] != 0]

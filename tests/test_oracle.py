@@ -185,3 +185,80 @@ def test_threaded_runner_matches_single_queries():
         exp = O.expected(g, k, s) if k != 2 else O.wcc_lp(g)
         assert np.array_equal(out[i], exp)
         assert m[i] == (g.E if k == 2 else O.reached_edges(g, exp))
+
+
+def planted_variants(g, s, b, d, lab):
+    """(kind, row) pairs of wrong answers: a BFS depth / SSSP distance off by one
+    either way, a reachable vertex left unreached, an unreachable vertex given
+    a finite value, a second zero, and a wrong component label."""
+    out = []
+    reach = np.flatnonzero((b != INF) & (b > 0))
+    v = int(reach[len(reach) // 2])
+    for delta in (-1, 1):
+        bb = b.copy()
+        bb[v] = np.uint32(int(bb[v]) + delta)
+        out.append((0, bb))
+        dd = d.copy()
+        dd[v] = np.uint32(int(dd[v]) + delta)
+        out.append((1, dd))
+    for k, base in ((0, b), (1, d)):
+        x = base.copy()
+        x[v] = INF
+        out.append((k, x))
+        x = base.copy()
+        x[v] = 0
+        out.append((k, x))
+    unreach = np.flatnonzero(b == INF)
+    if len(unreach):
+        for k, base in ((0, b), (1, d)):
+            x = base.copy()
+            x[int(unreach[0])] = 5
+            out.append((k, x))
+    ll = lab.copy()
+    ll[v] = v if lab[v] != v else lab[v] + 1
+    out.append((2, ll))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_c_checkers_accept_oracle_and_reject_planted_mistakes(seed):
+    import cgp_amd
+    h = cgp_amd.HostCSR.rmat(10, 4, seed=seed + 1, wmin=1, wmax=64)  # sparse: several components
+    g = O.G.of(h)
+    s = int(h.sample_sources(1, seed)[0])
+    b, d, lab = O.bfs(g, s), O.sssp(g, s), O.wcc_uf(g)
+    assert O.check_sssp(g, s, b, unit=True) is None
+    assert O.check_sssp(g, s, d) is None
+    assert O.check_wcc(g, lab) is None
+    assert np.any(b == INF), "the planted unreachable case needs a second component"
+    for kind, row in planted_variants(g, s, b, d, lab):
+        err = O.check_wcc(g, row) if kind == 2 else O.check_sssp(g, s, row, unit=(kind == 0))
+        assert err is not None, kind
+    # The threaded row checker agrees with the single-row checkers.
+    rows = [(0, b), (1, d), (2, lab)] + planted_variants(g, s, b, d, lab)
+    bad = O.check_rows(g, [k for k, _ in rows], [s] * len(rows), np.stack([r for _, r in rows]), threads=3)
+    assert [i for i, _ in bad] == list(range(3, len(rows)))
+
+
+def test_wcc_checkers_reject_merged_components():
+    # Components {0} and {1, 2}: labelling every vertex 0 satisfies "equal across
+    # edges", "label <= id" and "label is a fixed point", but is wrong.
+    g = O.G([0, 0, 1, 2], [2, 1])
+    wrong = np.zeros(3, dtype=np.uint32)
+    assert O.check_wcc(g, wrong) is not None
+    assert check_wcc(g.off, g.idx, wrong) is not None
+    right = np.array([0, 1, 1], dtype=np.uint32)
+    assert np.array_equal(O.wcc_uf(g), right)
+    assert O.check_wcc(g, right) is None and check_wcc(g.off, g.idx, right) is None
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_c_checkers_on_directed_random_graphs(seed):
+    rng = np.random.default_rng(seed + 40)
+    g = random_graph(rng, V=60, E=120, wmax=9, symmetric=False, self_loops=True)
+    for s in range(0, 60, 7):
+        assert O.check_sssp(g, s, O.sssp(g, s)) is None
+        assert O.check_sssp(g, s, O.bfs(g, s), unit=True) is None
+        assert O.check_sssp(g, s, O.bellman_ford(g, s)) is None
+    lab = O.wcc_uf(g)
+    assert O.check_wcc(g, lab) is None and check_wcc(g.off, g.idx, lab) is None
