@@ -14,7 +14,8 @@ import cgp_amd
 import oracle_lib as O
 from cgp_amd import BFS, SSSP, WCC
 
-pytestmark = pytest.mark.gpu
+# Full-size graphs take minutes (generation, 32 GiB result copies, threaded checks).
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
 
 def say(*a):
