@@ -269,6 +269,18 @@ class Engine:
         check(lib.cgp_run_batch(self._h, q, n, out.ptr, 1, C.byref(st)))
         return None, st.as_dict()
 
+    def run_batch_static(self, kinds, sources, out):
+        """Allocation-free run (cgp_run_batch_static): every per-run device
+        buffer comes from the library's fixed device array; ``out`` is a
+        DeviceBuffer (or view) that receives the n x V results."""
+        q = make_queries(kinds, sources)
+        n = len(q)
+        if out.nbytes < n * self.graph.V * 4:
+            raise ValueError("device output buffer too small")
+        st = cgp_run_stats()
+        check(lib.cgp_run_batch_static(self._h, q, n, out.ptr, C.byref(st)))
+        return st.as_dict()
+
     def last_run_report(self):
         """Per-iteration records of the last run_batch (list of dicts)."""
         n = C.c_int32(0)

@@ -159,6 +159,7 @@ SIGNATURES = {
     "cgp_engine_destroy": (None, [_p]),
     "cgp_engine_get_opts": (_st, [_p, C.POINTER(cgp_engine_opts)]),
     "cgp_run_batch": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.c_int32, C.POINTER(cgp_run_stats)]),
+    "cgp_run_batch_static": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.POINTER(cgp_run_stats)]),
     "cgp_reached_edges": (_st, [_p, _p, C.c_int32, _u64p]),
     "cgp_last_run_report": (_st, [_p, C.POINTER(cgp_iteration_record), C.c_int32, _i32p]),
     "cgp_engine_autotune": (_st, [_p, C.POINTER(cgp_query), C.c_int32, C.POINTER(cgp_engine_opts)]),

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -2012,7 +2013,7 @@ void compute_wcc(cgp_engine* en) {
     en->wcc_probe.ensure(kProbe * 4);
     hipLaunchKernelGGL(k_wcc_probe, dim3(kProbe / BLOCK), dim3(BLOCK), 0, en->s, (const int32_t*)parent, g->V,
                        kProbe, (int32_t*)en->wcc_probe.p);
-    std::vector<int32_t> probe(kProbe);
+    std::array<int32_t, kProbe> probe;
     HIP_CHECK(hipMemcpyAsync(probe.data(), en->wcc_probe.p, kProbe * 4, hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
     std::sort(probe.begin(), probe.end());
@@ -2043,9 +2044,9 @@ void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) 
 // Runs the frontier iterations of one pass to convergence: replays of a
 // captured K-iteration graph (or direct launches), one host readback of the
 // control block per replay.
-static void run_iterations(cgp_engine* en, int W) {
+static void run_iterations(cgp_engine* en, int W, bool no_graph) {
   const KP P = en->kp();
-  const bool use_graph = en->opts.use_hip_graph != 0;
+  const bool use_graph = en->opts.use_hip_graph != 0 && !no_graph;
   const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
   cgp_engine::GExec& gx = en->gx[wlog(W)];
   const int ls = launch_set(en);
@@ -2111,12 +2112,17 @@ void add_work_bytes(cgp_run_stats& st, const u64* pf, int W) {
 }
 
 void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
-                      bool out_dev, cgp_run_stats* stats) {
+                      bool out_dev, cgp_run_stats* stats, bool no_alloc) {
   const cgp_graph* g = en->g;
   const int32_t V = g->V;
   if (n <= 0) fail(CGP_EINVAL, "cgp_run_batch: batch must contain at least one query");
   if (!queries || !out) fail(CGP_EINVAL, "cgp_run_batch: null queries or output");
-  std::vector<int32_t> bfs, sssp, wcc;
+  std::vector<int32_t>& bfs = en->q_bfs;
+  std::vector<int32_t>& sssp = en->q_sssp;
+  std::vector<int32_t>& wcc = en->q_wcc;
+  bfs.clear();
+  sssp.clear();
+  wcc.clear();
   for (int32_t i = 0; i < n; ++i) {
     const cgp_query& q = queries[i];
     if (q.kind != CGP_BFS && q.kind != CGP_SSSP && q.kind != CGP_WCC)
@@ -2147,9 +2153,9 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   // SSSP slots fill the fewest words (value rows of BFS-only words are never
   // read by the advance).
   const int slots_max = std::max(1, std::min(en->opts.max_words, MAXW)) * 64;
-  std::vector<int32_t> order(sssp);
+  std::vector<int32_t>& order = en->q_order;
+  order.assign(sssp.begin(), sssp.end());
   order.insert(order.end(), bfs.begin(), bfs.end());
-  std::vector<double> prof_sum(NPROF, 0.0);
   for (size_t p0 = 0; p0 < order.size(); p0 += slots_max) {
     const int ns = (int)std::min<size_t>(slots_max, order.size() - p0);
     const int Wused = (ns + 63) / 64;
@@ -2233,14 +2239,14 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->next_dirty = false;
     dispatch_seed(en, W, P, (const int32_t*)en->slot_src.p, W * 64);
     en->tick(CAT_INIT, false);
-    run_iterations(en, W);
+    run_iterations(en, W, no_alloc);
     const Ctl& hc = *en->h_ctl;
     st.iterations += (int32_t)hc.iter;
     st.push_iterations += (int32_t)hc.push_iters;
     st.pull_iterations += (int32_t)hc.pull_iters;
     st.passes += 1;
     if (en->counted()) add_work_bytes(st, hc.prof, W);
-    {
+    if (!no_alloc) {
       // Per-iteration records of this pass (the ring keeps the last ITER_RING).
       const size_t nrec = std::min<size_t>(hc.iter, (size_t)ITER_RING);
       const size_t first = report.size();
@@ -2300,6 +2306,156 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     }
   }
   if (stats) *stats = st;
+}
+
+// ------------------------------------------------------------ no-allocation runs
+// Every per-run device buffer of cgp_run_batch_static lives in this array:
+// CGP_STATIC_SCRATCH_BYTES per GPU, held from the library's load until the
+// process ends.  Host code never uses the symbol itself as an address (in
+// host code it is only a stand-in); static_scratch() looks up each device's
+// own copy once with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[CGP_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;  // serialises static runs of this process (one array per GPU)
+void* g_static_ptr[kMaxDevices] = {};
+
+// Address of the calling device's copy of `scratch` (looked up on first use
+// per device; the caller holds g_static_mu).
+void* static_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) fail(CGP_EDEVICE, "cgp_run_batch_static: device index out of range");
+  if (!g_static_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      fail(CGP_EDEVICE, std::string("cgp_run_batch_static: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_static_ptr[dev] = p;
+  }
+  return g_static_ptr[dev];
+}
+
+// Byte layout of the per-run buffers inside the array, for Wm mask words, Rm
+// lanes per value row and nw WCC rows (each region 256-byte aligned).
+struct StaticLayout {
+  size_t visited, front, dense[2], dense_n[2], mark, touched, small[2], medium[2], large[2], vals, parent,
+      wcc_rows, wcc_probe, slot_src, qmap, total;
+};
+
+StaticLayout static_layout(const cgp_graph* g, int Wm, int Rm, int nw) {
+  const size_t V = (size_t)g->V, E = (size_t)g->E;
+  StaticLayout L{};
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  L.visited = take(V * Wm * 16);
+  L.front = take(V * Wm * 8);
+  for (int k = 0; k < 2; ++k) {
+    L.dense[k] = take((size_t)Wm * (V + E / DCHUNK + 1) * 8);
+    L.dense_n[k] = take((size_t)Wm * (V + E / DCHUNK + 1) * 8);
+  }
+  L.mark = take(V * 4 + 4);
+  L.touched = take(V * 4 + 4);
+  for (int k = 0; k < 2; ++k) {
+    L.small[k] = take(V * 4 + 4);
+    L.medium[k] = take(V * 4 + 4);
+    L.large[k] = take((V + E / LCHUNK + 1) * 8);
+  }
+  L.vals = take((size_t)Wm * V * Rm * 4);
+  L.parent = take(V * 4 + 4);
+  L.wcc_rows = take((size_t)std::max(nw, 1) * 4);
+  L.wcc_probe = take(1024 * 4);
+  L.slot_src = take((size_t)Wm * 64 * 4);
+  L.qmap = take((size_t)Wm * 64 * 4);
+  L.total = at;
+  return L;
+}
+
+}  // namespace
+
+void engine_run_batch_static(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
+                             cgp_run_stats* stats) {
+  const cgp_graph* g = en->g;
+  if (n <= 0 || n > CGP_STATIC_MAX_QUERIES)
+    fail(CGP_ECAPACITY, "cgp_run_batch_static: batch of " + std::to_string(n) + " queries (1 to " +
+                            std::to_string(CGP_STATIC_MAX_QUERIES) + " allowed)");
+  if (!queries || !out) fail(CGP_EINVAL, "cgp_run_batch_static: null queries or output");
+  // The widest pass: words and value-row lanes as engine_run_batch picks them.
+  int nv = 0, nw = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (queries[i].kind == CGP_WCC) ++nw;
+    else ++nv;
+  }
+  const int slots_max = std::max(1, std::min(en->opts.max_words, MAXW)) * 64;
+  const int first = std::min(nv, slots_max);
+  int Wm = 1, Rm = 1;
+  if (first > 0) {
+    Wm = round_words((first + 63) / 64);
+    while (Rm < std::min(64, first)) Rm <<= 1;
+    if (first > 64) Rm = 64;
+  }
+  const StaticLayout L = static_layout(g, Wm, Rm, nw);
+  if (L.total > (size_t)CGP_STATIC_SCRATCH_BYTES)
+    fail(CGP_ECAPACITY, "cgp_run_batch_static: this batch needs " + std::to_string(L.total) +
+                            " bytes of device scratch; the static array holds " +
+                            std::to_string((size_t)CGP_STATIC_SCRATCH_BYTES) +
+                            " (use cgp_run_batch for larger graphs or batches)");
+  std::lock_guard<std::mutex> glk(g_static_mu);
+  int cur = 0;
+  HIP_CHECK(hipGetDevice(&cur));
+  if (cur != en->dev) HIP_CHECK(hipSetDevice(en->dev));
+  struct RestoreDev {
+    int d;
+    bool on;
+    ~RestoreDev() {
+      if (on) (void)hipSetDevice(d);
+    }
+  } restore_dev{cur, cur != en->dev};
+  unsigned char* base = (unsigned char*)static_scratch(en->dev);
+
+  // Bind every per-run buffer to its region of the array for this call,
+  // and put the engine's own buffers back afterwards (also on failure).
+  DevBuf* bufs[] = {&en->visited, &en->front, &en->dense[0], &en->dense[1], &en->dense_n[0], &en->dense_n[1],
+                    &en->mark, &en->touched, &en->small[0], &en->small[1], &en->medium[0], &en->medium[1],
+                    &en->large[0], &en->large[1], &en->vals, &en->parent, &en->wcc_rows, &en->wcc_probe,
+                    &en->slot_src, &en->qmap};
+  const size_t offs[] = {L.visited, L.front, L.dense[0], L.dense[1], L.dense_n[0], L.dense_n[1],
+                         L.mark, L.touched, L.small[0], L.small[1], L.medium[0], L.medium[1],
+                         L.large[0], L.large[1], L.vals, L.parent, L.wcc_rows, L.wcc_probe,
+                         L.slot_src, L.qmap};
+  constexpr int NB = (int)(sizeof(offs) / sizeof(offs[0]));
+  struct Saved {
+    cgp_engine* en;
+    DevBuf own[NB];
+    DevBuf* const* bufs;
+    int wcap, profile;
+    u32 stamp;
+    ~Saved() {
+      for (int k = 0; k < NB; ++k) *bufs[k] = own[k];
+      en->wcap = wcap;
+      en->opts.profile = profile;
+      en->stamp_epoch = stamp;
+    }
+  } saved{en, {}, bufs, en->wcap, en->opts.profile, en->stamp_epoch};
+  for (int k = 0; k < NB; ++k) {
+    saved.own[k] = *bufs[k];
+    const size_t end = k + 1 < NB ? offs[k + 1] : L.total;
+    bufs[k]->p = base + offs[k];
+    bufs[k]->bytes = end - offs[k];
+    bufs[k]->borrowed = true;
+  }
+  en->wcap = Wm;
+  en->opts.profile &= ~1;  // no event timing (events would be created on demand)
+  // The array keeps the previous call's contents (any engine on this GPU):
+  // the vertex stamps must start below this call's first stamp.  The other
+  // regions are cleared or written by the run before they are read.
+  fill_u32(en->s, en->mark.p, (size_t)g->V + 1, 0u, en->cus);
+  en->stamp_epoch = 1;
+  engine_run_batch(en, queries, n, out, true, stats, true);
 }
 
 }  // namespace cgpi
@@ -2521,6 +2677,7 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
       int dev = 0;
       int khz = 0;
       HIP_CHECK(hipGetDevice(&dev));
+      en->dev = dev;
       if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
         en->clock_khz = (double)khz;
     }
@@ -2534,6 +2691,11 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     HIP_CHECK(hipHostMalloc((void**)&en->h_cfg, sizeof(Cfg), hipHostMallocDefault));
     HIP_CHECK(hipMemset(en->d_ctl, 0, sizeof(Ctl)));
     en->iters.ensure(sizeof(IterRec) * (size_t)ITER_RING);
+    // Host staging of slot sources, query map and WCC rows, and the query
+    // lists, sized so that batches of up to CGP_STATIC_MAX_QUERIES queries
+    // (every cgp_run_batch_static call) allocate no host memory.
+    en->stage(std::max<size_t>((size_t)MAXW * 64 * 2, CGP_STATIC_MAX_QUERIES));
+    for (auto* v : {&en->q_bfs, &en->q_sssp, &en->q_wcc, &en->q_order}) v->reserve(CGP_STATIC_MAX_QUERIES);
     *out = en.release();
   });
 }
@@ -2575,6 +2737,16 @@ cgp_status cgp_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, ui
     std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
     if (!lk.owns_lock() || en->serve) cgpi::fail(CGP_EBUSY, "cgp_run_batch: engine is busy");
     cgpi::engine_run_batch(en, queries, n, out, out_on_device != 0, stats);
+  });
+}
+
+cgp_status cgp_run_batch_static(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out_device,
+                                cgp_run_stats* stats) {
+  CGP_TRY({
+    if (!en) cgpi::fail(CGP_EINVAL, "cgp_run_batch_static: null engine");
+    std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
+    if (!lk.owns_lock() || en->serve) cgpi::fail(CGP_EBUSY, "cgp_run_batch_static: engine is busy");
+    cgpi::engine_run_batch_static(en, queries, n, out_device, stats);
   });
 }
 

@@ -54,14 +54,17 @@ namespace cgpi {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;  // a view into memory the buffer does not own (never freed or grown)
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
   }
   // Grows to at least `need` bytes; returns true when the address changed.
   bool ensure(size_t need) {
     if (need <= bytes) return false;
+    if (borrowed) cgpi::fail(CGP_ECAPACITY, "internal: borrowed device buffer too small");
     release();
     if (need == 0) return false;
     HIP_CHECK(hipMalloc(&p, need));
@@ -99,6 +102,10 @@ struct ServeState;
 struct cgp_engine {
   const cgp_graph* g = nullptr;
   cgp_engine_opts opts{};
+  int dev = 0;  // HIP device current at creation (the engine's streams live there)
+  // Host-side query lists of a batch run, reserved at creation so that runs of
+  // up to CGP_STATIC_MAX_QUERIES queries do not allocate host memory.
+  std::vector<int32_t> q_bfs, q_sssp, q_wcc, q_order;
   hipStream_t s = nullptr;
   hipStream_t s2 = nullptr;  // second stream of a concurrent advance (default priority)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -266,8 +273,15 @@ void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus);
 // Narrow-word mask of a slot configuration (all SSSP words narrow, else 0).
 u32 narrow_mask(const Cfg& cfg, int W);
 // One batched run (all passes, WCC, output transpose); stats may be null.
+// no_alloc: launches iterations directly (no graph capture), records no
+// per-iteration report and no event timing, and grows no buffer (the caller
+// has bound every per-run buffer beforehand).
 void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out, bool out_dev,
-                      cgp_run_stats* stats);
+                      cgp_run_stats* stats, bool no_alloc = false);
+// The same run with every per-run device buffer inside the library's static
+// device array (engine.hip); out is device memory.
+void engine_run_batch_static(cgp_engine* en, const cgp_query* queries, int32_t n, uint32_t* out,
+                             cgp_run_stats* stats);
 // Computes min-id component labels into en->parent (hooking + pointer jumping).
 void compute_wcc(cgp_engine* en);
 // Writes the labels into rows rows_out[0..nrows) of out (query-major, V per row).

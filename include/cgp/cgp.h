@@ -211,6 +211,24 @@ cgp_status cgp_engine_get_opts(const cgp_engine* e, cgp_engine_opts* out);
 cgp_status cgp_run_batch(cgp_engine* e, const cgp_query* queries, int32_t n, uint32_t* out,
                          int32_t out_on_device, cgp_run_stats* stats);
 
+/* Allocation-free batched run.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as cgp_run_batch on the same engine,
+ * with every per-run device buffer taken from one fixed device array built into the
+ * library (CGP_STATIC_SCRATCH_BYTES per GPU, held from library load until the process
+ * ends) instead of the engine's buffers; it calls no hipMalloc / hipFree /
+ * hipHostMalloc.  out_device: n*V uint32 of device memory, query-major.  Batches of
+ * 1..CGP_STATIC_MAX_QUERIES queries whose state fits the array (about V*(44 + 56*W +
+ * 4*W*R) + E*W/4 bytes for W mask words of R-lane value rows, R <= 64) run; larger
+ * ones fail with CGP_ECAPACITY and a message giving the bytes needed.  Iterations are
+ * launched directly (no graph capture); stats timing fields stay 0 and no
+ * per-iteration report is kept.  Every engine on one GPU shares the array: calls in
+ * one process are serialised, calls from two processes on one GPU race. */
+#define CGP_STATIC_SCRATCH_BYTES (64u << 20)
+#define CGP_STATIC_MAX_QUERIES 1024
+cgp_status cgp_run_batch_static(cgp_engine* e, const cgp_query* queries, int32_t n, uint32_t* out_device,
+                                cgp_run_stats* stats);
+
 /* One record per frontier iteration of the last cgp_run_batch, written on the
  * device by the compaction kernel (no host sync); replaces iteration_profile_t /
  * level_profile_t (puercgp/include/puercgp/core/result.hxx:21-46,
