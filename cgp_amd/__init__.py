@@ -177,6 +177,17 @@ class Graph:
         check(lib.cgp_graph_create_csr(len(o) - 1, len(i), op, ip, wp, C.byref(out)))
         return cls(out.value)
 
+    @classmethod
+    def from_device(cls, n_vertices, n_edges, offsets, indices, weights=None):
+        """Non-owning graph over device arrays (DeviceBuffer or view objects
+        with a ``ptr``); they must outlive the graph."""
+        out = C.c_void_p()
+        check(lib.cgp_graph_create_csr_device(int(n_vertices), int(n_edges), offsets.ptr, indices.ptr,
+                                              weights.ptr if weights is not None else None, C.byref(out)))
+        g = cls(out.value)
+        g._borrowed = (offsets, indices, weights)  # keep the arrays alive
+        return g
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.cgp_graph_destroy(self._h)
@@ -193,6 +204,15 @@ class DeviceBuffer:
         self.ptr = C.c_void_p()
         self.nbytes = int(nbytes)
         check(lib.cgp_device_alloc(self.nbytes, C.byref(self.ptr)))
+
+    @classmethod
+    def from_array(cls, a):
+        """A device copy of a contiguous numpy array."""
+        a = np.ascontiguousarray(a)
+        b = cls(max(1, a.nbytes))
+        if a.nbytes:
+            check(lib.cgp_copy_to_device(b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return b
 
     def to_host(self, dtype=np.uint32, shape=None):
         a = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
@@ -245,6 +265,17 @@ class Engine:
         o = cgp_engine_opts()
         check(lib.cgp_engine_get_opts(self._h, C.byref(o)))
         return {name: getattr(o, name) for name, _ in o._fields_}
+
+    def set_opts(self, **kw):
+        """Options for the following runs (cgp_engine_set_opts): fields not
+        named keep the engine's current values."""
+        o = cgp_engine_opts()
+        check(lib.cgp_engine_get_opts(self._h, C.byref(o)))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise AttributeError("unknown engine option %r" % k)
+            setattr(o, k, v)
+        check(lib.cgp_engine_set_opts(self._h, C.byref(o)))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

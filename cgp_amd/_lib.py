@@ -74,10 +74,13 @@ class cgp_run_stats(C.Structure):
         ("dense_ms", C.c_double),
         ("dense_launches", C.c_int64),
         ("dense_bytes", C.c_double),
+        ("work", C.c_uint64 * 24),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["work"] = [int(x) for x in self.work]
+        return d
 
 
 class cgp_iteration_record(C.Structure):
@@ -150,6 +153,7 @@ SIGNATURES = {
     "cgp_poisson_arrivals": (_st, [C.c_int32, C.c_double, C.c_uint64, _dp]),
     "cgp_graph_create_csr": (_st, [C.c_int32, C.c_int64, _i32p, _i32p, _u32p, _pp]),
     "cgp_graph_from_host": (_st, [_p, _pp]),
+    "cgp_graph_create_csr_device": (_st, [C.c_int32, C.c_int64, _p, _p, _p, _pp]),
     "cgp_graph_destroy": (None, [_p]),
     "cgp_graph_num_vertices": (C.c_int32, [_p]),
     "cgp_graph_num_edges": (C.c_int64, [_p]),
@@ -158,6 +162,7 @@ SIGNATURES = {
     "cgp_engine_create": (_st, [_p, C.POINTER(cgp_engine_opts), _pp]),
     "cgp_engine_destroy": (None, [_p]),
     "cgp_engine_get_opts": (_st, [_p, C.POINTER(cgp_engine_opts)]),
+    "cgp_engine_set_opts": (_st, [_p, C.POINTER(cgp_engine_opts)]),
     "cgp_run_batch": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.c_int32, C.POINTER(cgp_run_stats)]),
     "cgp_run_batch_static": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.POINTER(cgp_run_stats)]),
     "cgp_reached_edges": (_st, [_p, _p, C.c_int32, _u64p]),

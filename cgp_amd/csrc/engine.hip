@@ -212,6 +212,8 @@ struct Tally {
   u32 items = 0, edges = 0, wedges = 0, bfs = 0, nnew = 0, imp = 0;
   u32 ditems = 0, dedges = 0, pverts = 0, pwords = 0, pedges = 0;
   u64 sssp = 0;  // (edge, query) pairs: can pass 2^32 per lane
+  u32 swords = 0;  // (edge, word with sparse SSSP bits) pairs
+  u64 src = 0;     // (item, active SSSP query) pairs
   u32 fmin = INF;  // min value of pairs postponed or improved to >= thr
   u32 nnear = 0;   // improvements to a value < thr
 };
@@ -233,12 +235,23 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
   bool first = false;
   const size_t vb = vis_at<W>(v, 0);
   // All W visited and W next words of v in one go (one contiguous record),
-  // issued before any of them is used.
-  u64 nx[W], vis[W];
+  // issued before any of them is used.  In dense mode an edge with no BFS
+  // bit needs neither: its improved SSSP bits go to next[v] by a
+  // fire-and-forget atomicOr, so the record's line is touched only for an
+  // improvement, not for every edge scanned.
+  bool any_b = false;
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    nx[w] = P.next[vb + W + w];
-    vis[w] = P.visited[vb + w];
+  for (int w = 0; w < W; ++w) any_b |= bm[w] != 0;
+  u64 nx[W], vis[W];
+  if (SPARSE || any_b) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      nx[w] = P.next[vb + W + w];
+      vis[w] = P.visited[vb + w];
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w) nx[w] = vis[w] = 0;
   }
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -250,6 +263,7 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
     }
     u64 s = sm[w];
     if (s) {
+      ++t.swords;
       const size_t ub = vrow(P.V, w, u, rshift), vr = vrow(P.V, w, v, rshift);
       while (s) {
         // Up to 4 active queries per round, their 8 value loads in flight together.
@@ -374,7 +388,11 @@ __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0,
 #pragma unroll
   for (int w = 0; w < W; ++w) any_b |= (bm[w] != 0);
   if (!any_b && !any_s) return;
-  if (first == e0) ++t.items;
+  if (first == e0) {
+    ++t.items;
+#pragma unroll
+    for (int w = 0; w < W; ++w) t.src += (u64)__popcll(sm[w]);
+  }
   for (int32_t e = first; e < e1; e += stride) {
     const int32_t v = P.idx[e];
     u32 wv = 1;
@@ -438,7 +456,11 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
   const bool work = have && (any_b || any_s);
   const int32_t e0 = work ? P.off[u] : 0;
   const u32 d = work ? (u32)(P.off[u + 1] - e0) : 0u;
-  if (work) ++t.items;
+  if (work) {
+    ++t.items;
+#pragma unroll
+    for (int w = 0; w < W; ++w) t.src += (u64)__popcll(sm[w]);
+  }
   const u32 incl = wave_incl_scan(d);
   // Wave-uniform (scalar) so the loops below branch on the scalar unit and
   // the per-step row indices stay in scalar registers.
@@ -556,6 +578,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
     ++t.ditems;
     t.dedges += d;
     t.sssp += (u64)d * (u64)__popcll(sm);
+    t.src += (u64)__popcll(sm);
   }
   const u32 incl = wave_incl_scan(d);
   // Wave-uniform (scalar) so the loops below branch on the scalar unit and
@@ -701,6 +724,7 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
     ++t.ditems;
     t.dedges += dlen;
     t.sssp += (u64)dlen * (u64)__popcll(sm);
+    t.src += (u64)__popcll(sm);
   }
   const bool inrow = lane < R;
   const int lq = lane & (R - 1);
@@ -880,6 +904,7 @@ __device__ __forceinline__ void expand_flat_narrow(const KP& P, u32 base, u32 n,
     ++t.ditems;
     t.dedges += d;
     t.sssp += (u64)d * (u64)__popc(sm);
+    t.src += (u64)__popc(sm);
   }
   const u32 incl = wave_incl_scan(d);
   const u32 total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
@@ -1181,6 +1206,8 @@ __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
   prof_add(P, PF_PULL_VERTS, t.pverts);
   prof_add(P, PF_PULL_WORDS, t.pwords);
   prof_add(P, PF_PULL_EDGES, t.pedges);
+  prof_add(P, PF_SSSP_WORDS, t.swords);
+  prof_add(P, PF_SSSP_SRC, t.src);
 }
 
 // The advance of one iteration is two persistent launches of BS-thread
@@ -1228,6 +1255,7 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
     prof_add(P, PF_DENSE_PAIRS, t.sssp);
     prof_add(P, PF_DENSE_IMP, t.imp);
     prof_add(P, PF_DENSE_NEW, t.nnew);
+    prof_add(P, PF_DENSE_SRC, t.src);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1267,6 +1295,7 @@ __global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
     prof_add(P, PF_DENSE_PAIRS, t.sssp);
     prof_add(P, PF_DENSE_IMP, t.imp);
     prof_add(P, PF_DENSE_NEW, t.nnew);
+    prof_add(P, PF_DENSE_SRC, t.src);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1720,6 +1749,74 @@ __global__ __launch_bounds__(BLOCK) void k_fill_rows(const int32_t* off, int32_t
     for (int32_t e = off[v] + lane_id(); e < off[v + 1]; e += 64) rows[e] = (int32_t)v;
 }
 
+// ------------------------------------------------- graphs over caller arrays
+// Checks of a caller's device CSR before the engine reads it, and the
+// derived per-graph arrays built on the device.  Check flags: bit 0 offsets
+// decrease somewhere, bit 1 a column index is out of [0, V), bit 2 some row
+// is not sorted, bit 3 some edge has no reverse edge (only tested on sorted
+// rows: a binary search in the target's row).
+enum : u32 { CSR_BAD_OFFSETS = 1, CSR_BAD_INDEX = 2, CSR_UNSORTED = 4, CSR_ASYMMETRIC = 8 };
+
+__global__ __launch_bounds__(BLOCK) void k_csr_check_offsets(const int32_t* off, int32_t V, u32* flags) {
+  bool bad = false;
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK)
+    bad |= off[v + 1] < off[v];
+  if (__ballot(bad) && lane_id() == 0) atomicOr(flags, (u32)CSR_BAD_OFFSETS);
+}
+
+// Edge-parallel over slots whose source row `rows` is known (offsets valid).
+__global__ __launch_bounds__(BLOCK) void k_csr_check_edges(const int32_t* rows, const int32_t* idx, int64_t E,
+                                                           int32_t V, u32* flags) {
+  u32 f = 0;
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * BLOCK) {
+    const int32_t x = idx[e];
+    if (x < 0 || x >= V) f |= CSR_BAD_INDEX;
+    if (e > 0 && rows[e - 1] == rows[e] && idx[e - 1] > x) f |= CSR_UNSORTED;
+  }
+  if (__ballot(f != 0)) {
+    u32 m = f;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m |= (u32)__shfl_xor((int)m, d, 64);
+    if (lane_id() == 0) atomicOr(flags, m);
+  }
+}
+
+// Every edge u -> v has v -> u (rows sorted, indices in range: checked before).
+__global__ __launch_bounds__(BLOCK) void k_csr_check_symmetric(const int32_t* off, const int32_t* rows,
+                                                               const int32_t* idx, int64_t E, u32* flags) {
+  bool miss = false;
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * BLOCK) {
+    const int32_t u = rows[e], v = idx[e];
+    int32_t lo = off[v], hi = off[v + 1];
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (idx[mid] < u) lo = mid + 1;
+      else hi = mid;
+    }
+    miss |= !(lo < off[v + 1] && idx[lo] == u);
+  }
+  if (__ballot(miss) && lane_id() == 0) atomicOr(flags, (u32)CSR_ASYMMETRIC);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sum_u32(const u32* x, int64_t n, u64* sum) {
+  u64 s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) s += x[i];
+  s = wave_sum64(s);
+  if (lane_id() == 0 && s) atomicAdd(sum, s);
+}
+
+// Vertices with degree above PULL_WAVE_DEG, and those with 1..PULL_WAVE_DEG,
+// into two lists (counts in cnt[0] (lo) and cnt[1] (hi)).
+__global__ __launch_bounds__(BLOCK) void k_degree_split(const int32_t* off, int32_t V, int32_t* lo, int32_t* hi,
+                                                        u32* cnt) {
+  for (int64_t b = (int64_t)blockIdx.x * BLOCK; b < V; b += (int64_t)gridDim.x * BLOCK) {
+    const int64_t v = b + threadIdx.x;
+    const int32_t d = v < V ? off[v + 1] - off[v] : 0;
+    wave_append<int32_t>(d > PULL_WAVE_DEG, (int32_t)v, &cnt[1], hi);
+    wave_append<int32_t>(d > 0 && d <= PULL_WAVE_DEG, (int32_t)v, &cnt[0], lo);
+  }
+}
+
 // --------------------------------------------------------------- WCC
 // Hooking plus pointer jumping over parent pointers with parent[x] <= x at all
 // times: a hook links the larger of two roots under the smaller with a
@@ -2107,6 +2204,8 @@ void add_work_bytes(cgp_run_stats& st, const u64* pf, int W) {
   st.dense_bytes += (double)pf[PF_DENSE_ITEMS] * 16.0 + (double)pf[PF_DENSE_EDGES] * 8.0 +
                     (double)pf[PF_DENSE_PAIRS] * 4.0 + (double)pf[PF_DENSE_IMP] * 4.0 +
                     (double)pf[PF_DENSE_NEW] * 16.0;
+  static_assert(NPROF <= CGP_WORK_COUNTERS, "cgp_run_stats.work holds every work counter");
+  for (int k = 0; k < NPROF; ++k) st.work[k] += pf[k];
   st.frontier_vertices += (int64_t)(pf[PF_PUSH_ITEMS] + pf[PF_DENSE_ITEMS]);
   st.edges_scanned += (int64_t)(pf[PF_PUSH_EDGES] + pf[PF_DENSE_EDGES] + pf[PF_PULL_EDGES]);
 }
@@ -2578,14 +2677,94 @@ cgp_status cgp_graph_from_host(const cgp_host_csr* h, cgp_graph** out) {
                               h->g.wt.empty() ? nullptr : h->g.wt.data(), out);
 }
 
+cgp_status cgp_graph_create_csr_device(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,
+                                       const uint32_t* wt, cgp_graph** out) {
+  CGP_TRY({
+    if (!out) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: null output");
+    *out = nullptr;
+    if (V < 0 || E < 0 || E >= INT32_MAX) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: bad sizes");
+    if (!off || (E > 0 && !idx)) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: null CSR array");
+    require_device();
+    // Device graph owning only its derived arrays; the CSR stays the caller's.
+    struct Owner {
+      cgp_graph* g;
+      u32* d_tmp = nullptr;
+      ~Owner() {
+        if (d_tmp) (void)hipFree(d_tmp);
+        if (g) cgp_graph_destroy(g);
+      }
+    } own{new cgp_graph()};
+    cgp_graph* g = own.g;
+    g->borrowed = true;
+    g->V = V;
+    g->E = E;
+    g->off = const_cast<int32_t*>(off);
+    g->idx = const_cast<int32_t*>(idx);
+    g->wt = const_cast<u32*>(wt);
+    g->weighted = wt != nullptr;
+    int32_t ends[2];
+    ends[0] = ends[1] = 0;
+    HIP_CHECK(hipMemcpy(&ends[0], off, 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(&ends[1], off + V, 4, hipMemcpyDeviceToHost));
+    if (ends[0] != 0 || ends[1] != E) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: offsets do not span [0, E]");
+    // Scratch: flags, degree-list counters, weight sum.
+    HIP_CHECK(hipMalloc((void**)&own.d_tmp, 32));
+    HIP_CHECK(hipMemset(own.d_tmp, 0, 32));
+    u32* flags = own.d_tmp;
+    u32* cnt = own.d_tmp + 2;
+    u64* wsum = (u64*)(own.d_tmp + 4);
+    const int cus = device_cus();
+    const dim3 gb(cus * 16);
+    auto read_flags = [&]() {
+      u32 f = 0;
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpy(&f, flags, 4, hipMemcpyDeviceToHost));
+      return f;
+    };
+    if (V > 0) hipLaunchKernelGGL(k_csr_check_offsets, gb, dim3(BLOCK), 0, 0, off, V, flags);
+    if (read_flags() & CSR_BAD_OFFSETS) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: offsets not monotone");
+    HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
+    if (V > 0) hipLaunchKernelGGL(k_fill_rows, gb, dim3(BLOCK), 0, 0, off, V, g->rows);
+    if (E > 0) hipLaunchKernelGGL(k_csr_check_edges, gb, dim3(BLOCK), 0, 0, (const int32_t*)g->rows, idx, E, V, flags);
+    u32 f = read_flags();
+    if (f & CSR_BAD_INDEX) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: column index out of range");
+    // Symmetry is only recognised on sorted rows (otherwise the graph is
+    // treated as directed: every edge hooks in WCC, and BFS never pulls).
+    if (!(f & CSR_UNSORTED) && E > 0)
+      hipLaunchKernelGGL(k_csr_check_symmetric, gb, dim3(BLOCK), 0, 0, off, (const int32_t*)g->rows, idx, E, flags);
+    f = read_flags();
+    g->symmetric = !(f & (CSR_UNSORTED | CSR_ASYMMETRIC));
+    if (wt && E > 0) {
+      hipLaunchKernelGGL(k_sum_u32, gb, dim3(BLOCK), 0, 0, wt, E, wsum);
+      u64 sum = 0;
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpy(&sum, wsum, 8, hipMemcpyDeviceToHost));
+      g->mean_weight = (double)sum / (double)E;
+    }
+    HIP_CHECK(hipMalloc((void**)&g->pull_lo, (size_t)V * 4 + 4));
+    HIP_CHECK(hipMalloc((void**)&g->pull_hi, (size_t)V * 4 + 4));
+    if (V > 0) hipLaunchKernelGGL(k_degree_split, gb, dim3(BLOCK), 0, 0, off, V, g->pull_lo, g->pull_hi, cnt);
+    u32 c[2];
+    c[0] = c[1] = 0;
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(c, cnt, 8, hipMemcpyDeviceToHost));
+    g->n_pull_lo = (int32_t)c[0];
+    g->n_pull_hi = (int32_t)c[1];
+    *out = g;
+    own.g = nullptr;
+  });
+}
+
 void cgp_graph_destroy(cgp_graph* g) {
   if (!g) return;
-  (void)hipFree(g->off);
-  (void)hipFree(g->idx);
+  if (!g->borrowed) {
+    (void)hipFree(g->off);
+    (void)hipFree(g->idx);
+    if (g->wt) (void)hipFree(g->wt);
+  }
   (void)hipFree(g->rows);
   (void)hipFree(g->pull_lo);
   (void)hipFree(g->pull_hi);
-  if (g->wt) (void)hipFree(g->wt);
   delete g;
 }
 
@@ -2617,6 +2796,71 @@ void cgp_engine_opts_default(cgp_engine_opts* o) {
   o->hbm_budget_bytes = 64ull << 30;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Options in force for graph g from the caller's `opts`: a zero (or
+// negative, where noted) tunable field takes its value from `base` (the
+// defaults at creation, the engine's current, possibly tuned, options
+// later); automatic fields are resolved for the graph.
+cgp_engine_opts resolve_opts(const cgp_graph* g, const cgp_engine_opts* opts, const cgp_engine_opts& base,
+                             const char* op) {
+  const std::string who(op);
+  cgp_engine_opts o = opts ? *opts : base;
+  if (opts) {
+    if (o.pull_frontier_ratio <= 0) o.pull_frontier_ratio = base.pull_frontier_ratio;
+    if (o.pull_edge_ratio <= 0) o.pull_edge_ratio = base.pull_edge_ratio;
+    if (o.max_words <= 0 || o.max_words > MAXW) o.max_words = base.max_words;
+    if (o.block_size <= 0) o.block_size = base.block_size;
+    if (o.bin_small <= 0) o.bin_small = base.bin_small;
+    if (o.bin_large <= 0) o.bin_large = base.bin_large;
+    if (o.graph_iterations <= 0) o.graph_iterations = base.graph_iterations;
+    if (o.hbm_budget_bytes == 0) o.hbm_budget_bytes = base.hbm_budget_bytes;
+    if (o.frontier_dense_div <= 0) o.frontier_dense_div = base.frontier_dense_div;
+    if (o.dense_unroll <= 0) o.dense_unroll = base.dense_unroll;
+    if (o.traversal_mode < 0 || o.traversal_mode > 2) cgpi::fail(CGP_EINVAL, who + ": bad traversal_mode");
+  }
+  if (o.bin_large < o.bin_small) o.bin_large = o.bin_small;
+  if (o.block_size != 128 && o.block_size != 256 && o.block_size != 512)
+    cgpi::fail(CGP_EINVAL, who + ": block_size must be 128, 256 or 512");
+  if (o.dense_threshold < 0) cgpi::fail(CGP_EINVAL, who + ": dense_threshold must be >= 0");
+  if (o.advance_blocks < 0) cgpi::fail(CGP_EINVAL, who + ": advance_blocks must be >= 0");
+  if (o.pull_max_queries < 0) cgpi::fail(CGP_EINVAL, who + ": pull_max_queries must be >= 0");
+  if (o.max_iterations < 0) cgpi::fail(CGP_EINVAL, who + ": max_iterations must be >= 0");
+  if (o.dense_unroll != 4 && o.dense_unroll != 8 && o.dense_unroll != 16)
+    cgpi::fail(CGP_EINVAL, who + ": dense_unroll must be 4, 8 or 16");
+  if (o.sssp_delta < -1) cgpi::fail(CGP_EINVAL, who + ": sssp_delta must be >= -1");
+  if (o.dense_wide_addr != 0 && o.dense_wide_addr != 1)
+    cgpi::fail(CGP_EINVAL, who + ": dense_wide_addr must be 0 or 1");
+  if (o.sssp_delta == -1) {
+    // Automatic near-far step.  Low-degree weighted graphs (grids, road
+    // networks: long shortest paths, many re-relaxations) get twenty mean
+    // weights (on a 4096^2 grid that is faster than no step both for one
+    // query and for 512); small-world graphs settle in few Bellman-Ford
+    // iterations and skip it.
+    const double deg = g->V > 0 ? (double)g->E / (double)g->V : 0.0;
+    o.sssp_delta =
+        (g->weighted && deg <= 8.0 && g->mean_weight > 1.0) ? (int32_t)std::lround(20.0 * g->mean_weight) : 0;
+  }
+  // A zero threshold keeps the base's (possibly tuned) value while the
+  // postponing regime it was chosen for is unchanged.
+  if (o.dense_threshold == 0 && opts && o.sssp_delta == base.sssp_delta) o.dense_threshold = base.dense_threshold;
+  if (o.dense_threshold == 0) {
+    // Automatic lane-per-query threshold.  The table loop (postponing off)
+    // costs less per edge row than the postponing loop, so it pays from
+    // fewer active queries: 6 took the RMAT-22 mixed batch from 68.3 to
+    // 66.5 ms, while on the 4096^2 grid (postponing on) 6 instead of 8
+    // took 2380 to 2535 ms.
+    o.dense_threshold = o.sssp_delta == 0 ? 6 : 8;
+  }
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
 cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cgp_engine** out) {
   CGP_TRY({
     if (!g || !out) cgpi::fail(CGP_EINVAL, "cgp_engine_create: null graph or output");
@@ -2626,52 +2870,7 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
     en->g = g;
     cgp_engine_opts d;
     cgp_engine_opts_default(&d);
-    if (opts) {
-      en->opts = *opts;
-      if (en->opts.pull_frontier_ratio <= 0) en->opts.pull_frontier_ratio = d.pull_frontier_ratio;
-      if (en->opts.pull_edge_ratio <= 0) en->opts.pull_edge_ratio = d.pull_edge_ratio;
-      if (en->opts.max_words <= 0 || en->opts.max_words > MAXW) en->opts.max_words = d.max_words;
-      if (en->opts.block_size <= 0) en->opts.block_size = d.block_size;
-      if (en->opts.bin_small <= 0) en->opts.bin_small = d.bin_small;
-      if (en->opts.bin_large <= 0) en->opts.bin_large = d.bin_large;
-      if (en->opts.graph_iterations <= 0) en->opts.graph_iterations = d.graph_iterations;
-      if (en->opts.hbm_budget_bytes == 0) en->opts.hbm_budget_bytes = d.hbm_budget_bytes;
-      if (en->opts.traversal_mode < 0 || en->opts.traversal_mode > 2)
-        cgpi::fail(CGP_EINVAL, "cgp_engine_create: bad traversal_mode");
-    } else {
-      en->opts = d;
-    }
-    if (en->opts.bin_large < en->opts.bin_small) en->opts.bin_large = en->opts.bin_small;
-    if (en->opts.block_size != 128 && en->opts.block_size != 256 && en->opts.block_size != 512)
-      cgpi::fail(CGP_EINVAL, "cgp_engine_create: block_size must be 128, 256 or 512");
-    if (en->opts.dense_threshold < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_threshold must be >= 0");
-    if (en->opts.frontier_dense_div <= 0) en->opts.frontier_dense_div = d.frontier_dense_div;
-    if (en->opts.dense_unroll <= 0) en->opts.dense_unroll = d.dense_unroll;
-    if (en->opts.advance_blocks < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: advance_blocks must be >= 0");
-    if (en->opts.pull_max_queries < 0) cgpi::fail(CGP_EINVAL, "cgp_engine_create: pull_max_queries must be >= 0");
-    if (en->opts.dense_unroll != 4 && en->opts.dense_unroll != 8 && en->opts.dense_unroll != 16)
-      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_unroll must be 4, 8 or 16");
-    if (en->opts.sssp_delta < -1) cgpi::fail(CGP_EINVAL, "cgp_engine_create: sssp_delta must be >= -1");
-    if (en->opts.dense_wide_addr != 0 && en->opts.dense_wide_addr != 1)
-      cgpi::fail(CGP_EINVAL, "cgp_engine_create: dense_wide_addr must be 0 or 1");
-    if (en->opts.sssp_delta == -1) {
-      // Automatic near-far step.  Low-degree weighted graphs (grids, road
-      // networks: long shortest paths, many re-relaxations) get twenty mean
-      // weights (on a 4096^2 grid that is faster than no step both for one
-      // query and for 512); small-world graphs settle in few Bellman-Ford
-      // iterations and skip it.
-      const double deg = g->V > 0 ? (double)g->E / (double)g->V : 0.0;
-      en->opts.sssp_delta =
-          (g->weighted && deg <= 8.0 && g->mean_weight > 1.0) ? (int32_t)std::lround(20.0 * g->mean_weight) : 0;
-    }
-    if (en->opts.dense_threshold == 0) {
-      // Automatic lane-per-query threshold.  The table loop (postponing off)
-      // costs less per edge row than the postponing loop, so it pays from
-      // fewer active queries: 6 took the RMAT-22 mixed batch from 68.3 to
-      // 66.5 ms, while on the 4096^2 grid (postponing on) 6 instead of 8
-      // took 2380 to 2535 ms.
-      en->opts.dense_threshold = en->opts.sssp_delta == 0 ? 6 : 8;
-    }
+    en->opts = resolve_opts(g, opts, d, "cgp_engine_create");
     en->cus = device_cus();
     {
       int dev = 0;
@@ -2727,6 +2926,23 @@ cgp_status cgp_engine_get_opts(const cgp_engine* en, cgp_engine_opts* out) {
   CGP_TRY({
     if (!en || !out) cgpi::fail(CGP_EINVAL, "cgp_engine_get_opts: null argument");
     *out = en->opts;
+  });
+}
+
+cgp_status cgp_engine_set_opts(cgp_engine* en, const cgp_engine_opts* opts) {
+  CGP_TRY({
+    if (!en || !opts) cgpi::fail(CGP_EINVAL, "cgp_engine_set_opts: null argument");
+    std::unique_lock<std::mutex> lk(en->mu, std::try_to_lock);
+    if (!lk.owns_lock() || en->serve) cgpi::fail(CGP_EBUSY, "cgp_engine_set_opts: engine is busy");
+    // Automatic fields resolve again from the caller's values (-1 / 0).
+    cgp_engine_opts base = en->opts;
+    const cgp_engine_opts o = resolve_opts(en->g, opts, base, "cgp_engine_set_opts");
+    if (std::memcmp(&o, &en->opts, sizeof(o)) != 0) {
+      // Captured iteration graphs bake in launch shapes and kernel
+      // parameters (workgroup size, bins, unroll, row addressing).
+      en->drop_graphs();
+      en->opts = o;
+    }
   });
 }
 

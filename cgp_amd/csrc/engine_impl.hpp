@@ -37,6 +37,7 @@ struct cgp_graph {
   int64_t E = 0;
   bool symmetric = false;
   bool weighted = false;
+  bool borrowed = false;  // off / idx / wt belong to the caller (cgp_graph_create_csr_device)
   double mean_weight = 1.0;  // over all edges (1 when unweighted)
   int32_t* off = nullptr;
   int32_t* idx = nullptr;

@@ -48,7 +48,7 @@ constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
 // RMAT-22 (80 SSSP) 1024 / 512 / 256 / 128 / 64 edges measured 37.2 / 32.1 /
 // 29.6 / 28.8 / 30.0 ms of lane-per-query time per batch.
 constexpr int DCHUNK = 128;
-constexpr int NPROF = 15;
+constexpr int NPROF = 18;
 constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
 
 // Per-batch slot configuration (device resident, rewritten per pass).
@@ -168,7 +168,10 @@ enum : int {
   PF_PULL_EDGES,       // edges scanned by pull
   PF_DENSE_PAIRS,      // (edge, active query) pairs of the dense launch
   PF_DENSE_IMP,        // values lowered by the dense launch
-  PF_DENSE_NEW         // next words gaining bits in the dense launch
+  PF_DENSE_NEW,        // next words gaining bits in the dense launch
+  PF_SSSP_WORDS,       // (edge, mask word with active sparse SSSP bits) pairs of the push launch
+  PF_SSSP_SRC,         // (frontier item, active sparse SSSP query) pairs of the push launch
+  PF_DENSE_SRC         // (dense item, active SSSP query) pairs
 };
 
 }  // namespace cgpk

@@ -16,7 +16,8 @@ static void check(const char* name, bool cond) {
   if (!cond) ++failures;
 }
 
-static std::vector<uint32_t> row(const cgp::result_t& r, int q) {
+static std::vector<uint32_t> row(const cgp::result_t& r, int q) { return r.query_to_host(q); }
+static std::vector<uint32_t> row(const cgp::host_result_t& r, int q) {
   return std::vector<uint32_t>(r.query(q), r.query(q) + r.n_vertices);
 }
 
@@ -71,6 +72,34 @@ int main(int argc, char** argv) {
     check("q1 SSSP [0,2,5,5]", row(rr, 1) == std::vector<uint32_t>({0, 2, 5, 5}));
     check("q2 BFS  [INF,0,INF,1]", row(rr, 2) == std::vector<uint32_t>({INF, 0, INF, 1}));
     check("q3 SSSP [INF,0,INF,3]", row(rr, 3) == std::vector<uint32_t>({INF, 0, INF, 3}));
+
+    // Per-call options (smoke_hybrid_push.cu:151-160): one level on T1, then
+    // the same context runs to convergence again.
+    cgp::run_options one;
+    one.max_iterations = 1;
+    auto r1 = cgp::run_heterogeneous(t1, {{0, cgp::algo_kind_t::bfs}, {0, cgp::algo_kind_t::sssp}}, c1, one);
+    std::printf("per-call options (max_iterations 1, then 0)\n");
+    check("BFS  one level [0,1,1,INF]", row(r1, 0) == std::vector<uint32_t>({0, 1, 1, INF}));
+    check("SSSP one level [0,2,5,INF]", row(r1, 1) == std::vector<uint32_t>({0, 2, 5, INF}));
+    auto r2 = cgp::bfs::run(t1, {0}, c1);
+    check("BFS  converged [0,1,1,2]", row(r2, 0) == std::vector<uint32_t>({0, 1, 1, 2}));
+
+    // A graph over device arrays (no copy), results into caller device memory.
+    {
+      cgp::device_values d_off(off.size()), d_idx(idx.size()), d_w(w.size()), d_out(8);
+      check("device CSR upload",
+            cgp_copy_to_device(d_off.data(), off.data(), off.size() * 4) == CGP_OK &&
+                cgp_copy_to_device(d_idx.data(), idx.data(), idx.size() * 4) == CGP_OK &&
+                cgp_copy_to_device(d_w.data(), w.data(), w.size() * 4) == CGP_OK);
+      cgp::graph gd(cgp::graph::device_arrays, 4, 3, (const int32_t*)d_off.data(), (const int32_t*)d_idx.data(),
+                    d_w.data());
+      cgp::context cd(gd);
+      cgp::run_heterogeneous_into(gd, {{0, cgp::algo_kind_t::bfs}, {0, cgp::algo_kind_t::sssp}}, cd, d_out.data());
+      std::printf("device-array graph, device output\n");
+      auto h = d_out.to_host();
+      check("BFS  [0,1,1,2]", std::vector<uint32_t>(h.begin(), h.begin() + 4) == std::vector<uint32_t>({0, 1, 1, 2}));
+      check("SSSP [0,2,5,3]", std::vector<uint32_t>(h.begin() + 4, h.end()) == std::vector<uint32_t>({0, 2, 5, 3}));
+    }
 
     // Error behaviour: an empty batch is std::invalid_argument.
     bool threw = false;

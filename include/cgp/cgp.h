@@ -112,6 +112,29 @@ typedef struct {
                                   they do not */
 } cgp_engine_opts;
 
+/* Indices of cgp_run_stats.work: counts of the advance launches' work units. */
+#define CGP_WORK_COUNTERS 24
+typedef enum {
+  CGP_WORK_PUSH_ITEMS = 0,  /* frontier items expanded lane-per-edge (vertex or edge chunk) */
+  CGP_WORK_PUSH_EDGES = 1,  /* edges scanned lane-per-edge */
+  CGP_WORK_PUSH_WEDGES = 2, /* ... of which read a weight (SSSP bits active) */
+  CGP_WORK_BFS_WORDS = 3,   /* (edge, active BFS mask word) visited reads */
+  CGP_WORK_SSSP_PAIRS = 4,  /* (edge, active sparse SSSP query) target value reads */
+  CGP_WORK_NEW_WORDS = 5,   /* next-frontier words that gained bits (push launch) */
+  CGP_WORK_IMPROVED = 6,    /* (vertex, query) values lowered (push launch) */
+  CGP_WORK_DENSE_ITEMS = 7, /* lane-per-query (vertex, word, edge chunk) items */
+  CGP_WORK_DENSE_EDGES = 8, /* edges scanned lane-per-query, one per (edge, active SSSP word) */
+  CGP_WORK_PULL_VERTS = 9,  /* vertices examined bottom-up */
+  CGP_WORK_PULL_WORDS = 10, /* (edge, word) reads bottom-up */
+  CGP_WORK_PULL_EDGES = 11, /* edges scanned bottom-up */
+  CGP_WORK_DENSE_PAIRS = 12,/* (edge, active SSSP query) pairs lane-per-query */
+  CGP_WORK_DENSE_IMP = 13,  /* values lowered lane-per-query */
+  CGP_WORK_DENSE_NEW = 14,  /* next-frontier words that gained bits lane-per-query */
+  CGP_WORK_SSSP_WORDS = 15, /* (edge, mask word with active sparse SSSP bits) pairs */
+  CGP_WORK_SSSP_SRC = 16,   /* (item, active sparse SSSP query) source value reads */
+  CGP_WORK_DENSE_SRC = 17   /* (lane-per-query item, active SSSP query) pairs */
+} cgp_work_counter;
+
 /* Per-run report; replaces run_result_t / iteration_profile_t
  * (puercgp/include/puercgp/core/result.hxx:13-66) and cgp_bfs::result_t timing fields. */
 typedef struct {
@@ -132,6 +155,9 @@ typedef struct {
   double dense_ms;             /* profile bit 0: summed time of the dense-SSSP advance launches */
   int64_t dense_launches;      /* profile bit 0: dense-SSSP advance launches that had work */
   double dense_bytes;          /* profile bit 1: algorithmic bytes of those launches */
+  uint64_t work[CGP_WORK_COUNTERS]; /* profile bit 1: raw advance work counters summed over the
+                                  passes, indexed by cgp_work_counter (the units both byte
+                                  models are computed from) */
 } cgp_run_stats;
 
 /* ---------------------------------------------------------------- errors */
@@ -189,6 +215,17 @@ cgp_status cgp_graph_create_csr(int32_t n_vertices, int64_t n_edges, const int32
                                 const int32_t* col_indices, const uint32_t* weights,
                                 cgp_graph** out);
 cgp_status cgp_graph_from_host(const cgp_host_csr* g, cgp_graph** out);
+/* Non-owning graph over caller-owned device (HBM) arrays; replaces the non-owning
+ * csr_graph_view (puercgp/include/puercgp/backend/csr_graph.hxx:9-20).  row_offsets (V+1),
+ * col_indices (E) and weights (E, or NULL for unit weights) are device pointers that must
+ * stay valid and unchanged while the graph lives; nothing is copied.  The library checks
+ * them on the device (offsets span [0, E] and never decrease, indices in [0, V):
+ * CGP_EINVAL otherwise) and builds its own derived arrays (source row of each edge slot,
+ * degree lists).  The graph counts as symmetric when every row is sorted and every edge
+ * has its reverse; otherwise it is treated as directed. */
+cgp_status cgp_graph_create_csr_device(int32_t n_vertices, int64_t n_edges, const int32_t* row_offsets,
+                                       const int32_t* col_indices, const uint32_t* weights,
+                                       cgp_graph** out);
 void cgp_graph_destroy(cgp_graph* g);
 int32_t cgp_graph_num_vertices(const cgp_graph* g);
 int64_t cgp_graph_num_edges(const cgp_graph* g);
@@ -201,6 +238,13 @@ cgp_status cgp_engine_create(const cgp_graph* g, const cgp_engine_opts* opts, cg
 void cgp_engine_destroy(cgp_engine* e);
 /* Options actually in force (after defaults and tuning). */
 cgp_status cgp_engine_get_opts(const cgp_engine* e, cgp_engine_opts* out);
+/* Options for the following runs of this engine: the per-call options of
+ * cgp_bfs::run(G, sources, ctx, options) (cgp/include/gunrock/algorithms/cgp_bfs.hxx:1050-1056)
+ * and puercgp::run(graph, batch, ctx, run_options) (puercgp/include/puercgp/run.hxx:11-23).
+ * A zero tunable field keeps the engine's current (tuned or default) value; automatic
+ * fields (sssp_delta -1, dense_threshold 0) resolve again for the graph.  Captured
+ * iteration graphs are dropped when the options change.  CGP_EBUSY while serving. */
+cgp_status cgp_engine_set_opts(cgp_engine* e, const cgp_engine_opts* opts);
 
 /* Batched run of n queries of any mix of kinds.
  * Replaces gunrock::cgp_bfs::run / cgp_sssp::run (cgp/include/gunrock/algorithms/cgp_bfs.hxx:1050-1056,

@@ -21,6 +21,7 @@
 #ifndef CGP_CGP_HPP
 #define CGP_CGP_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -132,6 +133,17 @@ class graph {
     check(cgp_graph_create_csr(V, E, row_offsets, column_indices, weights, &g));
     g_.reset(g);
   }
+  // Non-owning view over the caller's device (HBM) arrays, as csr_graph_view
+  // (puercgp/include/puercgp/backend/csr_graph.hxx:9-20): nothing is copied,
+  // and the arrays must outlive the graph.
+  struct device_arrays_t {};
+  static constexpr device_arrays_t device_arrays{};
+  graph(device_arrays_t, int32_t V, int64_t E, const int32_t* d_row_offsets, const int32_t* d_column_indices,
+        const uint32_t* d_weights = nullptr) {
+    cgp_graph* g = nullptr;
+    check(cgp_graph_create_csr_device(V, E, d_row_offsets, d_column_indices, d_weights, &g));
+    g_.reset(g);
+  }
   int get_number_of_vertices() const { return cgp_graph_num_vertices(g_.get()); }
   int64_t get_number_of_edges() const { return cgp_graph_num_edges(g_.get()); }
   bool is_symmetric() const { return cgp_graph_is_symmetric(g_.get()) != 0; }
@@ -168,9 +180,10 @@ struct run_options {
   int sssp_delta = -1;  // -1: automatic (engine default)
   int dense_wide_addr = 0;
 
+  // Unset tunable fields stay zero: at context creation they take the
+  // defaults, on a later call the context's current (possibly tuned) values.
   cgp_engine_opts to_c() const {
-    cgp_engine_opts o;
-    cgp_engine_opts_default(&o);
+    cgp_engine_opts o{};
     o.traversal_mode = (int32_t)traversal_mode;
     o.pull_frontier_ratio = pull_frontier_ratio;
     o.pull_edge_ratio = pull_edge_ratio;
@@ -207,6 +220,11 @@ class context {
   }
   cgp_engine* handle() const { return e_.get(); }
   const graph& g() const { return *g_; }
+  // Options for the following runs (the reference passes them per call).
+  void apply(const run_options& o) {
+    cgp_engine_opts co = o.to_c();
+    check(cgp_engine_set_opts(e_.get(), &co));
+  }
   // Per-graph tuning on a sample batch; the result sticks to this context.
   cgp_engine_opts autotune(const std::vector<query_descriptor_t>& sample) {
     std::vector<cgp_query> q = to_c(sample);
@@ -228,27 +246,75 @@ class context {
   std::unique_ptr<cgp_engine, Del> e_;
 };
 
-// result_t: query-major values [q * V + v] (cgp_bfs.hxx:131-149) plus run stats.
+// Device-resident uint32 array (the reference's results are
+// thrust::device_vector, cgp_bfs.hxx:133); copies to the host on request.
+class device_values {
+ public:
+  device_values() = default;
+  explicit device_values(size_t n) : n_(n) {
+    void* p = nullptr;
+    check(cgp_device_alloc((uint64_t)std::max<size_t>(n, 1) * 4, &p));
+    p_.reset(p);
+  }
+  uint32_t* data() { return (uint32_t*)p_.get(); }
+  const uint32_t* data() const { return (const uint32_t*)p_.get(); }
+  size_t size() const { return n_; }
+  // Elements [first, first + count) into host memory.
+  void copy_to_host(uint32_t* dst, size_t first, size_t count) const {
+    if (count) check(cgp_copy_to_host(dst, data() + first, (uint64_t)count * 4));
+  }
+  std::vector<uint32_t> to_host() const {
+    std::vector<uint32_t> h(n_);
+    copy_to_host(h.data(), 0, n_);
+    return h;
+  }
+
+ private:
+  struct Del {
+    void operator()(void* p) const { (void)cgp_device_free(p); }
+  };
+  std::unique_ptr<void, Del> p_;
+  size_t n_ = 0;
+};
+
+// result_t: query-major values [q * V + v] (cgp_bfs.hxx:131-149), resident in
+// HBM, plus run stats.
 struct result_t {
-  std::vector<uint32_t> distances;
+  device_values distances;
   int n_vertices = 0;
   int n_queries = 0;
   cgp_run_stats stats{};
   int iterations = 0;
   float wall_time_ms = 0;
-  const uint32_t* query(int q) const { return distances.data() + (size_t)q * (size_t)n_vertices; }
+  const uint32_t* query_device(int q) const { return distances.data() + (size_t)q * (size_t)n_vertices; }
+  std::vector<uint32_t> query_to_host(int q) const {
+    std::vector<uint32_t> h((size_t)n_vertices);
+    distances.copy_to_host(h.data(), (size_t)q * (size_t)n_vertices, (size_t)n_vertices);
+    return h;
+  }
 };
 
-inline result_t run_heterogeneous(const graph& G, const std::vector<query_descriptor_t>& batch,
-                                  context& ctx, const run_options& = {}) {
+// Runs a batch into caller-owned device memory d_out (n * V uint32) with
+// this call's options: no allocation and no host copy of the results.
+inline cgp_run_stats run_heterogeneous_into(const graph& G, const std::vector<query_descriptor_t>& batch,
+                                            context& ctx, uint32_t* d_out, const run_options& o = {}) {
   if (&ctx.g() != &G) throw invalid_argument(CGP_EINVAL, "run: context belongs to another graph");
+  if (batch.empty()) throw invalid_argument(CGP_EINVAL, "query batch must contain at least one source");
+  ctx.apply(o);
+  std::vector<cgp_query> q = context::to_c(batch);
+  cgp_run_stats st{};
+  check(cgp_run_batch(ctx.handle(), q.data(), (int32_t)q.size(), d_out, 1, &st));
+  return st;
+}
+
+inline result_t run_heterogeneous(const graph& G, const std::vector<query_descriptor_t>& batch,
+                                  context& ctx, const run_options& o = {}) {
+  if (batch.empty()) throw invalid_argument(CGP_EINVAL, "query batch must contain at least one source");
   result_t r;
   r.n_vertices = G.get_number_of_vertices();
   r.n_queries = (int)batch.size();
-  if (batch.empty()) throw invalid_argument(CGP_EINVAL, "query batch must contain at least one source");
-  r.distances.resize((size_t)r.n_queries * (size_t)r.n_vertices);
-  std::vector<cgp_query> q = context::to_c(batch);
-  check(cgp_run_batch(ctx.handle(), q.data(), (int32_t)q.size(), r.distances.data(), 0, &r.stats));
+  r.distances = device_values((size_t)r.n_queries * (size_t)r.n_vertices);
+  r.stats = run_heterogeneous_into(G, batch, ctx, r.distances.data(), o);
   r.iterations = r.stats.iterations;
   r.wall_time_ms = (float)r.stats.wall_ms;
   return r;
@@ -333,11 +399,20 @@ class server {
   context& ctx_;
 };
 
+// Host-resident results of the serving loop (each completed query's row is
+// copied to host memory while the traversal continues).
+struct host_result_t {
+  std::vector<uint32_t> distances;  // query-major [q * V + v]
+  int n_vertices = 0;
+  int n_queries = 0;
+  const uint32_t* query(int q) const { return distances.data() + (size_t)q * (size_t)n_vertices; }
+};
+
 // run_replenish_pipeline: all queries submitted at once through `slots`
 // concurrent slots; results in submission order, query-major.
-inline result_t run_replenish_pipeline(const graph& G, const std::vector<query_descriptor_t>& queries,
-                                       context& ctx, int slots = 64) {
-  result_t r;
+inline host_result_t run_replenish_pipeline(const graph& G, const std::vector<query_descriptor_t>& queries,
+                                            context& ctx, int slots = 64) {
+  host_result_t r;
   r.n_vertices = G.get_number_of_vertices();
   r.n_queries = (int)queries.size();
   r.distances.assign((size_t)r.n_queries * (size_t)r.n_vertices, infinity);
