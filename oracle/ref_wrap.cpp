@@ -9,9 +9,12 @@
 //   sssp_cpu::run                   cgp/examples/algorithms/sssp/sssp_cpu.hxx:22-72
 #include <thrust/host_vector.h>
 
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "mtx_loader.hxx"
@@ -91,6 +94,41 @@ void ref_cgp_sssp(int32_t V, const int32_t* off, const int32_t* idx, const uint3
   std::vector<int> p(static_cast<size_t>(V));
   int s = src;
   sssp_cpu::run<cgp_csr<uint32_t>, int, int, uint32_t>(c, s, out, p.data());
+}
+
+// Times the reference's puercgp validators over a batch (the CPU baseline
+// of the benchmark): the graph is converted to the reference's host graph
+// once, outside the clock; `threads` workers take the queries in order, one
+// query per worker at a time (kinds: 0 cpu_bfs, 1 cpu_sssp, 2 cpu_wcc to
+// convergence).  Returns the wall seconds of the query runs; checksum gets
+// the sum over queries of the finite values' count (keeps the results live).
+double ref_run_queries(int32_t V, const int32_t* off, const int32_t* idx, const uint32_t* wt,
+                       const int32_t* kinds, const int32_t* sources, int32_t n, int32_t threads,
+                       uint64_t* checksum) {
+  const auto g = make_puer(V, off, idx, wt);
+  std::atomic<int32_t> next{0};
+  std::atomic<uint64_t> sum{0};
+  auto worker = [&]() {
+    for (int32_t i = next++; i < n; i = next++) {
+      uint64_t c = 0;
+      if (kinds[i] == 0) {
+        for (int d : puercgp_examples::cpu_bfs(g, sources[i])) c += d != std::numeric_limits<int>::max();
+      } else if (kinds[i] == 1) {
+        for (float d : puercgp_examples::cpu_sssp(g, sources[i])) c += !std::isinf(d);
+      } else {
+        for (float l : puercgp_examples::cpu_wcc(g, std::numeric_limits<int>::max())) c += l >= 0.0f;
+      }
+      sum += c;
+    }
+  };
+  const int nt = threads < 1 ? 1 : threads;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (checksum) *checksum = sum.load();
+  return secs;
 }
 
 }  // extern "C"

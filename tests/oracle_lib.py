@@ -50,6 +50,10 @@ def _load_ref():
     lib.ref_puer_wcc.argtypes = [C.c_int32, _i32p, _i32p, C.c_int32, _u32p]
     lib.ref_cgp_bfs.argtypes = [C.c_int32, _i32p, _i32p, C.c_int32, _u32p]
     lib.ref_cgp_sssp.argtypes = [C.c_int32, _i32p, _i32p, _u32p, C.c_int32, _u32p]
+    if hasattr(lib, "ref_run_queries"):  # absent from a build older than its source
+        lib.ref_run_queries.argtypes = [C.c_int32, _i32p, _i32p, _u32p, _i32p, _i32p, C.c_int32, C.c_int32,
+                                        _u64p]
+        lib.ref_run_queries.restype = C.c_double
     return lib
 
 
@@ -142,6 +146,18 @@ def run_queries(g, kinds, sources, threads, keep=False):
                                int(threads), out.ctypes.data_as(_u32p) if keep else None,
                                m.ctypes.data_as(_u64p))
     return secs, out, m
+
+
+def ref_run_queries(g, kinds, sources, threads):
+    """Times the reference's own CPU validators (oracle/_ref) on a batch:
+    returns (seconds, checksum = count of reached (query, vertex) pairs)."""
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    s = np.ascontiguousarray(sources, dtype=np.int32)
+    ck = np.zeros(1, dtype=np.uint64)
+    off, idx, wt = g.p()
+    secs = ref.ref_run_queries(g.V, off, idx, wt, k.ctypes.data_as(_i32p), s.ctypes.data_as(_i32p), len(k),
+                               int(threads), ck.ctypes.data_as(_u64p))
+    return secs, int(ck[0])
 
 
 def ref_call(name, g, *args):

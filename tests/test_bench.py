@@ -55,3 +55,23 @@ def test_pmc_traffic_without_calibration_rows_is_none(tmp_path):
     write_counters(tmp_path / "x_counter_collection.csv", "FETCH_SIZE",
                    [("void k_adv_dense<1, 256, false, 16, 1>(KP)", 10.0)])
     assert bench.pmc_traffic(str(tmp_path), "k_adv_dense<", 1, 64, 1, 64, 1, 1.0) == (None, None)
+
+
+def test_byte_models_from_work_counters():
+    # One launch's counters by hand: 10 push items, 100 edges (40 with a
+    # weight), 70 (edge, BFS word) reads, 90 (edge, SSSP query) pairs in 30
+    # (edge, SSSP word) pairs, 5 next words, 7 lowered values, 12 (item, SSSP
+    # query) sources; lane-per-query: 3 items, 50 edges, 400 pairs, 9 lowered,
+    # 2 next words, 20 (item, query) sources.  W = 4 mask words.
+    w = [0] * 24
+    w[bench.PUSH_ITEMS], w[bench.PUSH_EDGES], w[bench.PUSH_WEDGES] = 10, 100, 40
+    w[bench.BFS_WORDS], w[bench.SSSP_PAIRS], w[bench.SSSP_WORDS] = 70, 90, 30
+    w[bench.NEW_WORDS], w[bench.IMPROVED], w[bench.SSSP_SRC] = 5, 7, 12
+    w[bench.DENSE_ITEMS], w[bench.DENSE_EDGES], w[bench.DENSE_PAIRS] = 3, 50, 400
+    w[bench.DENSE_IMP], w[bench.DENSE_NEW], w[bench.DENSE_SRC] = 9, 2, 20
+    push, dense = bench.bytes_builder(w, 4)
+    assert push == 10 * 40 + 100 * 4 + 40 * 4 + 70 * 8 + 90 * 4 + 7 * 4 + 5 * 16
+    assert dense == 3 * 16 + 50 * 8 + 400 * 4 + 9 * 4 + 2 * 16
+    push, dense = bench.bytes_survey(w, 4)
+    assert push == 10 * 40 + 100 * 4 + 40 * 4 + (70 + 30) * 8 + 5 * 16 + 7 * 4 + 12 * 4
+    assert dense == 3 * 40 + 50 * 16 + 2 * 16 + 9 * 4 + 20 * 4
