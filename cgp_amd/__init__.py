@@ -23,6 +23,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _lib
 from ._lib import (BFS, CGP_EBUSY, CGP_ECAPACITY, CGP_EDEVICE, CGP_EINVAL, CGP_EIO, CGP_ENOMEM,
                    CGP_OK, HYBRID, INF, PULL, PUSH, SSSP, WCC, CgpError, cgp_completion,
                    cgp_engine_opts, cgp_iteration_record, cgp_query, cgp_run_stats, cgp_serve_opts,
@@ -132,16 +133,23 @@ class HostCSR:
         return cls._make(lib.cgp_host_csr_from_coo, int(n_vertices), len(s), sp, dp, wp)
 
     @classmethod
-    def load_gr(cls, path):
-        return cls._make(lib.cgp_load_gr, str(path).encode())
+    def load_gr(cls, path, unit_weights=False):
+        """Galois .gr v1; unit_weights ignores stored weights (the reference's
+        SSSP driver reads .gr graphs this way)."""
+        return cls._make(lib.cgp_load_graph, str(path).encode(), _lib.FORMAT_GR,
+                         _lib.LOAD_UNIT_WEIGHTS if unit_weights else 0)
 
     @classmethod
-    def load_matrix_market(cls, path):
-        return cls._make(lib.cgp_load_matrix_market, str(path).encode())
+    def load_matrix_market(cls, path, unit_weights=False, round_weights=False):
+        """Matrix Market coordinate file; values must be integers in [0, 2^32-1]
+        unless round_weights rounds them; unit_weights ignores them."""
+        flags = (_lib.LOAD_UNIT_WEIGHTS if unit_weights else 0) | (_lib.LOAD_ROUND_WEIGHTS if round_weights else 0)
+        return cls._make(lib.cgp_load_graph, str(path).encode(), _lib.FORMAT_MATRIX_MARKET, flags)
 
     @classmethod
-    def load_csr_directory(cls, path):
-        return cls._make(lib.cgp_load_csr_directory, str(path).encode())
+    def load_csr_directory(cls, path, unit_weights=False):
+        return cls._make(lib.cgp_load_graph, str(path).encode(), _lib.FORMAT_CSR_DIRECTORY,
+                         _lib.LOAD_UNIT_WEIGHTS if unit_weights else 0)
 
     def degrees(self):
         return np.diff(self.offsets.astype(np.int64))
@@ -311,6 +319,16 @@ class Engine:
         st = cgp_run_stats()
         check(lib.cgp_run_batch_static(self._h, q, n, out.ptr, C.byref(st)))
         return st.as_dict()
+
+    def last_run_report_text(self, fmt="json", dataset=""):
+        """The last batch run's report (cgp_last_run_report_text) as a string:
+        fmt "json" or "csv"."""
+        f = {"json": _lib.REPORT_JSON, "csv": _lib.REPORT_CSV}[fmt]
+        n = C.c_uint64(0)
+        check(lib.cgp_last_run_report_text(self._h, f, dataset.encode(), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib.cgp_last_run_report_text(self._h, f, dataset.encode(), buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
 
     def last_run_report(self):
         """Per-iteration records of the last run_batch (list of dicts)."""

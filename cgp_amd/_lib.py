@@ -23,6 +23,10 @@ STATUS_NAMES = {
 BFS, SSSP, WCC = 0, 1, 2
 PUSH, PULL, HYBRID = 0, 1, 2
 INF = 0xFFFFFFFF
+# Loader formats and flags, report formats (cgp.h).
+FORMAT_GR, FORMAT_MATRIX_MARKET, FORMAT_CSR_DIRECTORY = 0, 1, 2
+LOAD_UNIT_WEIGHTS, LOAD_ROUND_WEIGHTS = 1, 2
+REPORT_JSON, REPORT_CSV = 0, 1
 
 
 class cgp_query(C.Structure):
@@ -96,11 +100,19 @@ class cgp_iteration_record(C.Structure):
         ("next_bfs_pairs", C.c_int64),
         ("advance_ms", C.c_double),
         ("filter_ms", C.c_double),
+        ("next_pairs", C.c_int64),
+        ("next_vertices", C.c_int64),
+        ("next_query_edges", C.c_int64),
+        ("edges_scanned", C.c_int64),
+        ("dense_ms", C.c_double),
+        ("end_ms", C.c_double),
+        ("frontier_union", C.c_uint64 * 8),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["pass"] = d.pop("pass_")
+        d["frontier_union"] = [int(x) for x in self.frontier_union]
         return d
 
 
@@ -147,6 +159,7 @@ SIGNATURES = {
     "cgp_load_gr": (_st, [C.c_char_p, _pp]),
     "cgp_load_matrix_market": (_st, [C.c_char_p, _pp]),
     "cgp_load_csr_directory": (_st, [C.c_char_p, _pp]),
+    "cgp_load_graph": (_st, [C.c_char_p, C.c_int32, C.c_int32, _pp]),
     "cgp_host_csr_view": (_st, [_p, _i32p, _i64p, C.POINTER(_i32p), C.POINTER(_i32p), C.POINTER(_u32p)]),
     "cgp_host_csr_destroy": (None, [_p]),
     "cgp_sample_sources": (_st, [_p, C.c_int32, C.c_uint64, _i32p]),
@@ -167,6 +180,7 @@ SIGNATURES = {
     "cgp_run_batch_static": (_st, [_p, C.POINTER(cgp_query), C.c_int32, _p, C.POINTER(cgp_run_stats)]),
     "cgp_reached_edges": (_st, [_p, _p, C.c_int32, _u64p]),
     "cgp_last_run_report": (_st, [_p, C.POINTER(cgp_iteration_record), C.c_int32, _i32p]),
+    "cgp_last_run_report_text": (_st, [_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
     "cgp_engine_autotune": (_st, [_p, C.POINTER(cgp_query), C.c_int32, C.POINTER(cgp_engine_opts)]),
     "cgp_serve_start": (_st, [_p, C.POINTER(cgp_serve_opts)]),
     "cgp_serve_submit": (_st, [_p, C.POINTER(cgp_query), C.c_int64, C.c_double]),

@@ -56,27 +56,27 @@ cgp_status cgp_host_csr_from_coo(int32_t V, int64_t E, const int32_t* src, const
   });
 }
 
-cgp_status cgp_load_gr(const char* path, cgp_host_csr** out) {
-  CGP_TRY({
-    if (!path || !out) cgpi::fail(CGP_EINVAL, "cgp_load_gr: null argument");
-    *out = nullptr;
-    wrap_host(cgpi::load_gr(path), out);
-  });
-}
+cgp_status cgp_load_gr(const char* path, cgp_host_csr** out) { return cgp_load_graph(path, CGP_FORMAT_GR, 0, out); }
 
 cgp_status cgp_load_matrix_market(const char* path, cgp_host_csr** out) {
-  CGP_TRY({
-    if (!path || !out) cgpi::fail(CGP_EINVAL, "cgp_load_matrix_market: null argument");
-    *out = nullptr;
-    wrap_host(cgpi::load_matrix_market(path), out);
-  });
+  return cgp_load_graph(path, CGP_FORMAT_MATRIX_MARKET, 0, out);
 }
 
 cgp_status cgp_load_csr_directory(const char* path, cgp_host_csr** out) {
+  return cgp_load_graph(path, CGP_FORMAT_CSR_DIRECTORY, 0, out);
+}
+
+cgp_status cgp_load_graph(const char* path, int32_t format, int32_t flags, cgp_host_csr** out) {
   CGP_TRY({
-    if (!path || !out) cgpi::fail(CGP_EINVAL, "cgp_load_csr_directory: null argument");
+    if (!path || !out) cgpi::fail(CGP_EINVAL, "cgp_load_graph: null argument");
     *out = nullptr;
-    wrap_host(cgpi::load_csr_directory(path), out);
+    if (flags & ~(CGP_LOAD_UNIT_WEIGHTS | CGP_LOAD_ROUND_WEIGHTS)) cgpi::fail(CGP_EINVAL, "cgp_load_graph: unknown flags");
+    switch (format) {
+      case CGP_FORMAT_GR: wrap_host(cgpi::load_gr(path, flags), out); break;
+      case CGP_FORMAT_MATRIX_MARKET: wrap_host(cgpi::load_matrix_market(path, flags), out); break;
+      case CGP_FORMAT_CSR_DIRECTORY: wrap_host(cgpi::load_csr_directory(path, flags), out); break;
+      default: cgpi::fail(CGP_EINVAL, "cgp_load_graph: unknown format " + std::to_string(format));
+    }
   });
 }
 

@@ -69,9 +69,10 @@ HostCSR gen_rmat(int scale, int edge_factor, double a, double b, double c, uint6
 HostCSR gen_grid(int rows, int cols, double p_extra, uint64_t seed, uint32_t wmin, uint32_t wmax);
 HostCSR csr_from_coo(int32_t V, int64_t E, const int32_t* src, const int32_t* dst,
                      const uint32_t* w);
-HostCSR load_gr(const std::string& path);
-HostCSR load_matrix_market(const std::string& path);
-HostCSR load_csr_directory(const std::string& path);
+// flags: CGP_LOAD_UNIT_WEIGHTS, CGP_LOAD_ROUND_WEIGHTS (include/cgp/cgp.h).
+HostCSR load_gr(const std::string& path, int32_t flags = 0);
+HostCSR load_matrix_market(const std::string& path, int32_t flags = 0);
+HostCSR load_csr_directory(const std::string& path, int32_t flags = 0);
 std::vector<int32_t> sample_sources(const HostCSR& g, int32_t n, uint64_t seed);
 bool is_symmetric(const HostCSR& g);
 

@@ -238,7 +238,11 @@ __device__ __forceinline__ bool relax_edge(const KP& P, int32_t u, int32_t v, u3
   // issued before any of them is used.  In dense mode an edge with no BFS
   // bit needs neither: its improved SSSP bits go to next[v] by a
   // fire-and-forget atomicOr, so the record's line is touched only for an
-  // improvement, not for every edge scanned.
+  // improvement, not for every edge scanned.  (A per-vertex byte of "BFS
+  // words every running query has visited", kept by the filter to skip the
+  // record read too, cut the visited tests of the RMAT-22 mixed batch by
+  // 39 % but not its traffic: the skipped records were cache hits, and the
+  // extra dependent load made 160 BFS 0.5 ms slower.)
   bool any_b = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) any_b |= bm[w] != 0;
@@ -1404,6 +1408,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   }
   const u32 depth_base = iter + 1u;
   u64 pairs = 0, epairs = 0, fedges = 0;
+  u64 apairs = 0, averts = 0, aqedges = 0;  // every kind: pairs, vertices, query-weighted degree
   // Each thread takes FK vertices per step (their loads issued together) and
   // the workgroup reserves their bin entries with one atomic per bin: the
   // bin counters are single addresses every workgroup contends for (one
@@ -1437,6 +1442,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       u64 nb[W];
       bool need_bin = false;
       u32 dmask = 0, nmask = 0, vpairs = 0;  // dense words (wide / narrow)
+      u32 allp = 0;                          // (v, query) pairs of every kind
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         const u64 nw = nwk[k][w];
@@ -1452,10 +1458,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
         if (nb[w]) need_bin = true;
         un[w] |= nw;
         vpairs += (u32)__popcll(nb[w]);
+        allp += (u32)__popcll(nw);
       }
       pairs += vpairs;
       epairs += (u64)vpairs * (u64)deg;
       fedges += (u64)deg;
+      apairs += allp;
+      averts += anyw ? 1u : 0u;
+      aqedges += (u64)allp * (u64)(anyw ? o1[k] - o0[k] : 0);
       if (deg == 0) {
         need_bin = false;
         dmask = 0;
@@ -1540,38 +1550,42 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   }
   // Frontier statistics: wave sums, then one atomic per workgroup and counter.
   {
-    __shared__ u64 s_st[(BLOCK / 64) * (3 + MAXW)];
+    constexpr int NS = 6;  // sums per wave before the union words
+    __shared__ u64 s_st[(BLOCK / 64) * (NS + MAXW)];
     const int wave = (int)(threadIdx.x >> 6);
-    pairs = wave_sum64(pairs);
-    epairs = wave_sum64(epairs);
-    fedges = wave_sum64(fedges);
+    u64 sums[NS] = {pairs, epairs, fedges, apairs, averts, aqedges};
+#pragma unroll
+    for (int i = 0; i < NS; ++i) sums[i] = wave_sum64(sums[i]);
     u64 unw[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) unw[w] = wave_or64(un[w]);
     if (lane == 0) {
-      u64* r = s_st + wave * (3 + MAXW);
-      r[0] = pairs;
-      r[1] = epairs;
-      r[2] = fedges;
+      u64* r = s_st + wave * (NS + MAXW);
 #pragma unroll
-      for (int w = 0; w < W; ++w) r[3 + w] = unw[w];
+      for (int i = 0; i < NS; ++i) r[i] = sums[i];
+#pragma unroll
+      for (int w = 0; w < W; ++w) r[NS + w] = unw[w];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      u64 tp = 0, te = 0, tf = 0, tu[W];
+      u64 tot[NS], tu[W];
+#pragma unroll
+      for (int i = 0; i < NS; ++i) tot[i] = 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) tu[w] = 0;
       for (int wv = 0; wv < BLOCK / 64; ++wv) {
-        const u64* r = s_st + wv * (3 + MAXW);
-        tp += r[0];
-        te += r[1];
-        tf += r[2];
+        const u64* r = s_st + wv * (NS + MAXW);
 #pragma unroll
-        for (int w = 0; w < W; ++w) tu[w] |= r[3 + w];
+        for (int i = 0; i < NS; ++i) tot[i] += r[i];
+#pragma unroll
+        for (int w = 0; w < W; ++w) tu[w] |= r[NS + w];
       }
-      if (tp) atomicAdd(&C->bfs_pairs[np], tp);
-      if (te) atomicAdd(&C->bfs_edges[np], te);
-      if (tf) atomicAdd(&C->fedges[np], tf);
+      if (tot[0]) atomicAdd(&C->bfs_pairs[np], tot[0]);
+      if (tot[1]) atomicAdd(&C->bfs_edges[np], tot[1]);
+      if (tot[2]) atomicAdd(&C->fedges[np], tot[2]);
+      if (tot[3]) atomicAdd(&C->npairs[np], tot[3]);
+      if (tot[4]) atomicAdd(&C->nverts[np], tot[4]);
+      if (tot[5]) atomicAdd(&C->nqedges[np], tot[5]);
 #pragma unroll
       for (int w = 0; w < W; ++w)
         if (tu[w]) atomicOr(&C->union_[w], tu[w]);
@@ -1592,9 +1606,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
                       atomicAdd(&C->bcount[np][4], 0u);
     const u64 fp = atomicAdd(&C->bfs_pairs[np], 0ull), fe = atomicAdd(&C->bfs_edges[np], 0ull);
     const u64 fall = atomicAdd(&C->fedges[np], 0ull);
+    const u64 napairs = atomicAdd(&C->npairs[np], 0ull), naverts = atomicAdd(&C->nverts[np], 0ull),
+              naqedges = atomicAdd(&C->nqedges[np], 0ull);
     C->dense_mode = fall >= cfg->dense_fedges_thr ? 1u : 0u;
     C->fedges[p] = 0;
+    C->npairs[p] = C->nverts[p] = C->nqedges[p] = 0;
+    u64 consumed[MAXW];
     for (int w = 0; w < MAXW; ++w) {
+      consumed[w] = C->last_union[w];
       C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
     }
     const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3] + C->bcount[p][4];
@@ -1651,7 +1670,20 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
         r.adv_ticks = at;
         r.flt_ticks = ft;
         r.den_ticks = dt;
+        r.next_pairs = napairs;
+        r.next_vertices = naverts;
+        r.next_qedges = naqedges;
+        r.edges_scanned = C->prof[PF_PUSH_EDGES] + C->prof[PF_DENSE_EDGES] + C->prof[PF_PULL_EDGES];
+        r.t_end = (u64)wall_clock64();
+        for (int w = 0; w < MAXW; ++w) r.fr_union[w] = consumed[w];
       }
+    } else {
+      // The seeding filter: the sizes of frontier 0, and the run's clock origin.
+      C->seed_pairs = napairs;
+      C->seed_verts = naverts;
+      C->seed_edges = fall;
+      C->seed_qedges = naqedges;
+      C->t_start = (u64)wall_clock64();
     }
     C->adv_t0 = ~0ull;
     C->adv_t1 = 0;
@@ -2233,6 +2265,15 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   cgp_run_stats st{};
   std::vector<IterRec>& report = en->report;
   report.clear();
+  en->rep_seed.clear();
+  en->rep_wall_ms = 0;
+  if (!no_alloc) {
+    en->rep_queries.assign(queries, queries + n);
+    en->rep_slot.assign((size_t)n, -1);
+  } else {
+    en->rep_queries.clear();
+    en->rep_slot.clear();
+  }
   if (V == 0) {
     if (stats) *stats = st;
     return;
@@ -2287,6 +2328,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       const int w = k >> 6, q = k & 63;
       h_src[k] = queries[qi].source;
       h_qmap[k] = qi;
+      if (!no_alloc) en->rep_slot[(size_t)qi] = (int32_t)((p0 / slots_max) << 16 | w << 8 | q);
       if (queries[qi].kind == CGP_BFS) {
         cfg.bfs_bits[w] |= 1ull << q;
         ++nb;
@@ -2346,6 +2388,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     st.passes += 1;
     if (en->counted()) add_work_bytes(st, hc.prof, W);
     if (!no_alloc) {
+      en->rep_seed.push_back({hc.seed_pairs, hc.seed_verts, hc.seed_edges, hc.seed_qedges, hc.t_start});
       // Per-iteration records of this pass (the ring keeps the last ITER_RING).
       const size_t nrec = std::min<size_t>(hc.iter, (size_t)ITER_RING);
       const size_t first = report.size();
@@ -2390,6 +2433,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   }
   HIP_CHECK(hipStreamSynchronize(en->s));
   st.wall_ms = now_ms() - t0;
+  en->rep_wall_ms = st.wall_ms;
   if (!out_dev) {
     HIP_CHECK(hipMemcpy(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost));
   }
@@ -2974,7 +3018,8 @@ cgp_status cgp_last_run_report(const cgp_engine* en, cgp_iteration_record* out, 
     for (int32_t i = 0; i < m; ++i) {
       const IterRec& r = en->report[(size_t)i];
       cgp_iteration_record& o = out[i];
-      o.pass = (int32_t)(r.pull >> 16);
+      const int pass = (int)(r.pull >> 16);
+      o.pass = pass;
       o.iteration = (int32_t)r.iter;
       o.pull = (int32_t)(r.pull & 0xFFFF);
       o.dense = (int32_t)r.dense;
@@ -2985,8 +3030,283 @@ cgp_status cgp_last_run_report(const cgp_engine* en, cgp_iteration_record* out, 
       o.next_bfs_pairs = (int64_t)r.next_bfs_pairs;
       o.advance_ms = (double)r.adv_ticks / khz;
       o.filter_ms = (double)r.flt_ticks / khz;
+      o.next_pairs = (int64_t)r.next_pairs;
+      o.next_vertices = (int64_t)r.next_vertices;
+      o.next_query_edges = (int64_t)r.next_qedges;
+      // The work counters accumulate over a pass: this iteration's share.
+      const bool first = i == 0 || (int)(en->report[(size_t)i - 1].pull >> 16) != pass;
+      o.edges_scanned = (int64_t)(r.edges_scanned - (first ? 0ull : en->report[(size_t)i - 1].edges_scanned));
+      o.dense_ms = (double)r.den_ticks / khz;
+      const u64 t0 = (size_t)pass < en->rep_seed.size() ? en->rep_seed[(size_t)pass].t_start : r.t_end;
+      o.end_ms = r.t_end >= t0 ? (double)(r.t_end - t0) / khz : 0.0;
+      for (int w = 0; w < MAXW; ++w) o.frontier_union[w] = r.fr_union[w];
     }
     *n = cap > 0 ? m : (int32_t)en->report.size();
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Minimal text builders for the run report.
+struct Text {
+  std::string s;
+  void raw(const std::string& x) { s += x; }
+  void str(const std::string& x) {
+    s += '"';
+    for (char c : x) {
+      if (c == '"' || c == '\\') {
+        s += '\\';
+        s += c;
+      } else if ((unsigned char)c < 0x20) {
+        char b[8];
+        std::snprintf(b, sizeof b, "\\u%04x", (unsigned)(unsigned char)c);
+        s += b;
+      } else {
+        s += c;
+      }
+    }
+    s += '"';
+  }
+  void num(double x) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.6f", x);
+    s += b;
+  }
+  void num(int64_t x) { s += std::to_string(x); }
+  void num(uint64_t x) { s += std::to_string(x); }
+  void num(int32_t x) { s += std::to_string(x); }
+  void key(const char* k) {
+    str(k);
+    s += ':';
+  }
+  template <typename T>
+  void arr(const char* k, const std::vector<T>& v) {
+    key(k);
+    s += '[';
+    for (size_t i = 0; i < v.size(); ++i) {
+      if (i) s += ',';
+      num(v[i]);
+    }
+    s += ']';
+  }
+};
+
+const char* kind_name(int32_t k) { return k == CGP_BFS ? "bfs" : k == CGP_SSSP ? "sssp" : "wcc"; }
+
+std::string report_text(const cgp_engine* en, int32_t format, const char* dataset) {
+  const size_t nrec = en->report.size();
+  std::vector<cgp_iteration_record> rec(nrec);
+  int32_t got = 0;
+  if (cgp_last_run_report(en, rec.data(), (int32_t)nrec, &got) != CGP_OK) fail(CGP_EDEVICE, cgp_last_error());
+  const std::string ds = dataset ? dataset : "";
+  const cgp_graph* g = en->g;
+  const int32_t nq = (int32_t)en->rep_queries.size();
+  int kinds_present = 0;
+  for (const cgp_query& q : en->rep_queries) kinds_present |= 1 << q.kind;
+  const std::string prim = kinds_present == 1 ? "cgp_bfs" : kinds_present == 2 ? "cgp_sssp"
+                           : kinds_present == 4 ? "cgp_wcc" : "cgp_mixed";
+  // Per level (passes concatenated): frontier L is the seeding filter's output
+  // for L = 0, else the previous iteration's next frontier.
+  struct Level {
+    int pass, level;
+    bool pull, dense;
+    uint64_t fpairs, fverts, fedges, fqedges, out_pairs, scanned;
+    double adv, flt, den, end;
+  };
+  std::vector<Level> lv;
+  const bool counted = (en->opts.profile & 2) != 0;
+  for (size_t i = 0; i < nrec; ++i) {
+    const cgp_iteration_record& r = rec[i];
+    const bool first = i == 0 || rec[i - 1].pass != r.pass;
+    Level l{};
+    l.pass = r.pass;
+    l.level = r.iteration;
+    l.pull = r.pull != 0;
+    l.dense = r.dense != 0;
+    if (first) {
+      const auto& sd = en->rep_seed[(size_t)r.pass];
+      l.fpairs = sd.pairs;
+      l.fverts = sd.verts;
+      l.fedges = sd.edges;
+      l.fqedges = sd.qedges;
+    } else {
+      l.fpairs = (uint64_t)rec[i - 1].next_pairs;
+      l.fverts = (uint64_t)rec[i - 1].next_vertices;
+      l.fedges = (uint64_t)rec[i - 1].next_edges;
+      l.fqedges = (uint64_t)rec[i - 1].next_query_edges;
+    }
+    l.out_pairs = (uint64_t)r.next_pairs;
+    l.scanned = counted ? (uint64_t)r.edges_scanned : l.fedges;
+    l.adv = r.advance_ms;
+    l.flt = r.filter_ms;
+    l.den = r.dense_ms;
+    l.end = r.end_ms;
+    lv.push_back(l);
+  }
+  // Per query: the last level of its pass whose advance had its bit, and the
+  // time that level ended (WCC: computed apart, complete with the run).
+  std::vector<int32_t> comp_level((size_t)nq, 0);
+  std::vector<double> comp_ms((size_t)nq, en->rep_wall_ms);
+  for (int32_t q = 0; q < nq; ++q) {
+    const int32_t sl = en->rep_slot[(size_t)q];
+    if (sl < 0) continue;
+    const int pass = sl >> 16, w = (sl >> 8) & 0xFF, b = sl & 63;
+    for (const cgp_iteration_record& r : rec)
+      if (r.pass == pass && ((r.frontier_union[w] >> b) & 1ull)) {
+        comp_level[(size_t)q] = r.iteration;
+        comp_ms[(size_t)q] = r.end_ms;
+      }
+  }
+  Text t;
+  if (format == CGP_REPORT_CSV) {
+    t.raw("algorithm,dataset,q,source_or_query_file,iteration,range_name,level_mode,active_vertices,"
+          "processed_edges,input_frontier,output_frontier,actual_edge_count,virtual_edge_count,elapsed_ms\n");
+    std::string srcs;
+    for (int32_t q = 0; q < nq; ++q) {
+      if (q) srcs += ' ';
+      srcs += std::to_string(en->rep_queries[(size_t)q].source);
+    }
+    const std::string alg = prim.substr(4);
+    for (const Level& l : lv) {
+      const std::string range = "cgp:" + alg + ":q" + std::to_string(nq) + ":pass:" + std::to_string(l.pass) +
+                                ":iter:" + std::to_string(l.level);
+      auto row = [&](const std::string& name, uint64_t edges, double ms) {
+        t.raw(alg + "," + ds + "," + std::to_string(nq) + "," + srcs + "," + std::to_string(l.level) + "," + name +
+              "," + (l.pull ? "pull" : "push") + "," + std::to_string(l.fverts) + "," + std::to_string(edges) +
+              "," + std::to_string(l.fpairs) + "," + std::to_string(l.out_pairs) + "," + std::to_string(l.scanned) +
+              "," + std::to_string(l.fqedges) + ",");
+        t.num(ms);
+        t.raw("\n");
+      };
+      row(range, l.fedges, l.adv + l.flt);
+      row(range + ":" + (l.pull ? "pull" : "push"), l.scanned, l.adv);
+      if (l.den > 0) row(range + ":lane_per_query", l.scanned, l.den);
+      row(range + ":compact", l.fedges, l.flt);
+    }
+    return t.s;
+  }
+  char name[256] = {0};
+  int32_t cus = 0;
+  uint64_t hbm = 0;
+  (void)cgp_device_info(name, sizeof name, &cus, &hbm);
+  t.raw("{");
+  t.key("engine"); t.str("cgp_amd (MI355X, gfx950)"); t.raw(",");
+  t.key("primitive"); t.str(prim); t.raw(",");
+  t.key("graph_file"); t.str(ds); t.raw(",");
+  t.key("num_vertices"); t.num((int64_t)g->V); t.raw(",");
+  t.key("num_edges"); t.num((int64_t)g->E); t.raw(",");
+  t.key("symmetric"); t.raw(g->symmetric ? "true," : "false,");
+  {
+    std::vector<int64_t> srcs;
+    for (const cgp_query& q : en->rep_queries) srcs.push_back(q.source);
+    t.arr("srcs", srcs);
+    t.raw(",");
+  }
+  t.key("query_results");
+  t.raw("[");
+  for (int32_t q = 0; q < nq; ++q) {
+    if (q) t.raw(",");
+    t.raw("{");
+    t.key("query_id"); t.num(q); t.raw(",");
+    t.key("source"); t.num(en->rep_queries[(size_t)q].source); t.raw(",");
+    t.key("kind"); t.str(kind_name(en->rep_queries[(size_t)q].kind)); t.raw(",");
+    t.key("pass"); t.num((int32_t)(en->rep_slot[(size_t)q] < 0 ? -1 : en->rep_slot[(size_t)q] >> 16)); t.raw(",");
+    t.key("completion_level"); t.num(comp_level[(size_t)q]); t.raw(",");
+    t.key("completion_wall_time_ms"); t.num(comp_ms[(size_t)q]);
+    t.raw("}");
+  }
+  t.raw("],");
+  std::vector<uint64_t> fs, ufs, lec, alec, vlec;
+  std::vector<double> lw;
+  std::string modes;
+  double gpu_ms = 0;
+  for (size_t i = 0; i < lv.size(); ++i) {
+    const Level& l = lv[i];
+    fs.push_back(l.fpairs);
+    ufs.push_back(l.fverts);
+    lec.push_back(l.fedges);
+    alec.push_back(l.scanned);
+    vlec.push_back(l.fqedges);
+    lw.push_back(l.adv + l.flt);
+    gpu_ms += l.adv + l.flt;
+    modes += (i ? "," : "");
+    modes += l.pull ? "\"pull\"" : "\"push\"";
+    // Each pass ends with an empty frontier (or its iteration limit).
+    if (i + 1 == lv.size() || lv[i + 1].pass != l.pass) {
+      fs.push_back(l.out_pairs);
+      ufs.push_back((uint64_t)rec[i].next_vertices);
+    }
+  }
+  t.arr("frontier_sizes", fs); t.raw(",");
+  t.arr("unique_frontier_sizes", ufs); t.raw(",");
+  t.arr("level_edge_counts", lec); t.raw(",");
+  t.arr("actual_level_edge_counts", alec); t.raw(",");
+  t.arr("virtual_level_edge_counts", vlec); t.raw(",");
+  t.key("level_modes"); t.raw("[" + modes + "],");
+  t.arr("level_wall_times_ms", lw); t.raw(",");
+  t.key("iterations"); t.num((int64_t)lv.size()); t.raw(",");
+  t.key("passes"); t.num((int64_t)en->rep_seed.size()); t.raw(",");
+  t.key("gpu_time_ms"); t.num(gpu_ms); t.raw(",");
+  t.key("wall_time_ms"); t.num(en->rep_wall_ms); t.raw(",");
+  t.key("traversal_mode");
+  t.str(en->opts.traversal_mode == CGP_PUSH ? "push" : en->opts.traversal_mode == CGP_PULL ? "pull" : "hybrid");
+  t.raw(",");
+  t.key("effective_query_dim"); t.num((int64_t)std::min<size_t>(en->rep_queries.size(), (size_t)en->opts.max_words * 64)); t.raw(",");
+  t.key("pull_frontier_ratio"); t.num(en->opts.pull_frontier_ratio); t.raw(",");
+  t.key("pull_edge_ratio"); t.num(en->opts.pull_edge_ratio); t.raw(",");
+  t.key("actual_edge_counts_measured"); t.raw(counted ? "true," : "false,");
+  t.key("level_profiles");
+  t.raw("[");
+  for (size_t i = 0; i < lv.size(); ++i) {
+    const Level& l = lv[i];
+    if (i) t.raw(",");
+    t.raw("{");
+    t.key("pass"); t.num((int32_t)l.pass); t.raw(",");
+    t.key("level"); t.num((int32_t)l.level); t.raw(",");
+    t.key("mode"); t.str(l.pull ? "pull" : "push"); t.raw(",");
+    t.key("frontier_mode"); t.str(l.dense ? "dense" : "sparse"); t.raw(",");
+    t.key("frontier_size"); t.num(l.fpairs); t.raw(",");
+    t.key("edge_count"); t.num(l.fedges); t.raw(",");
+    t.key("unique_frontier_size"); t.num(l.fverts); t.raw(",");
+    t.key("actual_edge_count"); t.num(l.scanned); t.raw(",");
+    t.key("virtual_edge_count"); t.num(l.fqedges); t.raw(",");
+    t.key("level_wall_ms"); t.num(l.adv + l.flt); t.raw(",");
+    t.key("push_kernel_ms"); t.num(l.pull ? 0.0 : l.adv); t.raw(",");
+    t.key("lane_per_query_kernel_ms"); t.num(l.den); t.raw(",");
+    t.key("pull_kernel_ms"); t.num(l.pull ? l.adv : 0.0); t.raw(",");
+    t.key("compact_ms"); t.num(l.flt); t.raw(",");
+    t.key("count_sync_ms"); t.num(0.0);
+    t.raw("}");
+  }
+  t.raw("],");
+  t.key("device"); t.raw("{");
+  t.key("name"); t.str(name); t.raw(",");
+  t.key("compute_units"); t.num(cus); t.raw(",");
+  t.key("hbm_bytes"); t.num(hbm);
+  t.raw("}}");
+  return t.s;
+}
+
+}  // namespace
+
+extern "C" {
+
+cgp_status cgp_last_run_report_text(const cgp_engine* en, int32_t format, const char* dataset, char* buf,
+                                    uint64_t cap, uint64_t* len) {
+  CGP_TRY({
+    if (!en || !len || (cap > 0 && !buf)) cgpi::fail(CGP_EINVAL, "cgp_last_run_report_text: null argument");
+    if (format != CGP_REPORT_JSON && format != CGP_REPORT_CSV)
+      cgpi::fail(CGP_EINVAL, "cgp_last_run_report_text: unknown format");
+    if (en->rep_queries.empty()) cgpi::fail(CGP_EINVAL, "cgp_last_run_report_text: no batch run to report");
+    const std::string s = report_text(en, format, dataset);
+    *len = s.size();
+    if (cap > 0) {
+      const size_t k = std::min<size_t>(s.size(), (size_t)cap - 1);
+      std::memcpy(buf, s.data(), k);
+      buf[k] = 0;
+    }
   });
 }
 

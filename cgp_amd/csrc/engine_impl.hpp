@@ -115,6 +115,16 @@ struct cgp_engine {
   cgpi::DevBuf visited, front, mark, vals, touched, small[2], medium[2], large[2], dense[2], dense_n[2];  // visited: visited + next records
   cgpi::DevBuf iters;  // per-iteration record ring (ITER_RING entries)
   std::vector<cgpk::IterRec> report;  // records of the last batch run, all passes
+  // The rest of the last batch run's report: its queries, each query's slot
+  // (pass << 16 | word << 8 | bit, -1 for WCC), and per pass the sizes of
+  // frontier 0 and the device-clock origin.
+  struct PassSeed {
+    uint64_t pairs, verts, edges, qedges, t_start;
+  };
+  std::vector<cgp_query> rep_queries;
+  std::vector<int32_t> rep_slot;
+  std::vector<PassSeed> rep_seed;
+  double rep_wall_ms = 0;
   cgpi::DevBuf parent, wcc_rows, wcc_probe, outbuf, qmap, slot_src;
   Ctl* d_ctl = nullptr;
   Cfg* d_cfg = nullptr;

@@ -215,7 +215,7 @@ HostCSR csr_from_coo(int32_t V, int64_t E, const int32_t* src, const int32_t* ds
 // is 4 the uint32 edge data follows after padding dst to 8 bytes.  The
 // reference ignores edge data (unit weights); this loader keeps 4-byte data
 // as weights when present and nonzero-sized.
-HostCSR load_gr(const std::string& path) {
+HostCSR load_gr(const std::string& path, int32_t flags) {
   std::ifstream in(path, std::ios::binary);
   if (!in) fail(CGP_EIO, "load_gr: could not open graph " + path);
   uint64_t hdr[4];
@@ -241,7 +241,7 @@ HostCSR load_gr(const std::string& path) {
     fail(CGP_EIO, "load_gr: failed reading GGR destinations");
   for (int64_t e = 0; e < g.E; ++e)
     if (g.idx[e] < 0 || g.idx[e] >= g.V) fail(CGP_EIO, "load_gr: destination out of range");
-  if (hdr[1] == 4) {
+  if (hdr[1] == 4 && !(flags & CGP_LOAD_UNIT_WEIGHTS)) {
     if (g.E & 1) in.ignore(4);
     g.wt.resize((size_t)g.E);
     if (!in.read(reinterpret_cast<char*>(g.wt.data()), (std::streamsize)(4 * (size_t)g.E)))
@@ -253,15 +253,19 @@ HostCSR load_gr(const std::string& path) {
 // Matrix Market coordinate (puercgp/examples/mtx_loader.hxx:99-187): 1-based
 // ids, "pattern" means unit weights, "symmetric" adds the reverse of every
 // off-diagonal entry; duplicates and self-loops are kept and the tuples are
-// sorted.  V = max(rows, cols).  Real weights are rounded to uint32.
-HostCSR load_matrix_market(const std::string& path) {
+// sorted.  V = max(rows, cols).  Weights are exact uint32 values: a
+// negative, fractional or too large value is an error (the reference keeps
+// float weights, which equal these integers only when they are integral)
+// unless CGP_LOAD_ROUND_WEIGHTS rounds it to the nearest integer in [0,
+// 2^32-1]; CGP_LOAD_UNIT_WEIGHTS ignores the values.
+HostCSR load_matrix_market(const std::string& path, int32_t flags) {
   std::ifstream in(path);
   if (!in) fail(CGP_EIO, "load_matrix_market: could not open matrix: " + path);
   std::string header;
   std::getline(in, header);
   std::string low = header;
   for (auto& ch : low) ch = (char)std::tolower((unsigned char)ch);
-  const bool pattern = low.find("pattern") != std::string::npos;
+  const bool pattern = low.find("pattern") != std::string::npos || (flags & CGP_LOAD_UNIT_WEIGHTS);
   const bool symmetric = low.find("symmetric") != std::string::npos;
   std::string line;
   do {
@@ -295,7 +299,11 @@ HostCSR load_matrix_market(const std::string& path) {
     --b;
     if (a < 0 || a >= V || b < 0 || b >= V)
       fail(CGP_EIO, "load_matrix_market: source vertex out of range in: " + path);
-    uint32_t wx = (uint32_t)std::llround(std::max(0.0, x));
+    if (!pattern && !(flags & CGP_LOAD_ROUND_WEIGHTS) &&
+        !(x >= 0.0 && x <= 4294967295.0 && x == std::floor(x)))
+      fail(CGP_EIO, "load_matrix_market: weight " + std::to_string(x) + " of entry " + std::to_string(i + 1) +
+                        " is not an integer in [0, 2^32-1] (CGP_LOAD_ROUND_WEIGHTS rounds it): " + path);
+    const uint32_t wx = (uint32_t)std::llround(std::min(4294967295.0, std::max(0.0, x)));
     s.push_back((int32_t)a);
     d.push_back((int32_t)b);
     w.push_back(wx);
@@ -310,7 +318,7 @@ HostCSR load_matrix_market(const std::string& path) {
 
 // CSR directory (puercgp/examples/mtx_loader.hxx:189-260): csr_vlist.bin
 // (int32[V+1]), csr_elist.bin (int32[E]), optional csr_weightlist.bin (int32[E]).
-HostCSR load_csr_directory(const std::string& path) {
+HostCSR load_csr_directory(const std::string& path, int32_t flags) {
   auto fsize = [](const std::string& p, int64_t* out) {
     struct stat st;
     if (stat(p.c_str(), &st) != 0 || !S_ISREG(st.st_mode)) return false;
@@ -337,7 +345,7 @@ HostCSR load_csr_directory(const std::string& path) {
     if (g.off[v + 1] < g.off[v]) fail(CGP_EIO, "load_csr_directory: offsets not monotone");
   for (int64_t e = 0; e < g.E; ++e)
     if (g.idx[e] < 0 || g.idx[e] >= g.V) fail(CGP_EIO, "load_csr_directory: column index out of range");
-  if (fsize(wp, &wb)) {
+  if (!(flags & CGP_LOAD_UNIT_WEIGHTS) && fsize(wp, &wb)) {
     if (wb != cb) fail(CGP_EIO, "load_csr_directory: csr_weightlist.bin size must equal csr_elist.bin size: " + path);
     std::vector<int32_t> wi((size_t)g.E);
     std::ifstream ws(wp, std::ios::binary);

@@ -102,6 +102,12 @@ struct Ctl {
   // over the launches of the run that did work.
   u64 adv_t0, adv_t1, flt_t0, den_t0, den_t1;
   u64 adv_ticks, flt_ticks, adv_launches, den_ticks, den_launches;
+  // Frontier statistics of the filter being run (all kinds): (vertex, query)
+  // pairs, distinct vertices and query-weighted degree of the next frontier;
+  // the seeding filter's values are kept as the sizes of frontier 0.
+  u64 npairs[2], nverts[2], nqedges[2];
+  u64 seed_pairs, seed_verts, seed_edges, seed_qedges;
+  u64 t_start;  // device clock at the end of the seeding filter
 };
 
 // Per-iteration record, written by the filter's last workgroup into a ring
@@ -119,6 +125,12 @@ struct IterRec {
   u64 adv_ticks;       // advance duration (both launches), device wall-clock ticks
   u64 flt_ticks;       // filter duration
   u64 den_ticks;       // dense-SSSP advance launch duration (0 when it had no work)
+  u64 next_pairs;      // (vertex, query) pairs of the next frontier, every kind
+  u64 next_vertices;   // distinct vertices of the next frontier
+  u64 next_qedges;     // sum over next-frontier pairs of the vertex's degree
+  u64 edges_scanned;   // edges scanned by the advances so far (profile counters; 0 without)
+  u64 t_end;           // device clock at this filter's end
+  u64 fr_union[MAXW];  // OR of the frontier words this iteration's advance consumed
 };
 
 // Static per-engine kernel parameters (buffer addresses and tuning).

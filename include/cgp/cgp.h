@@ -196,6 +196,18 @@ cgp_status cgp_host_csr_from_coo(int32_t n_vertices, int64_t n_edges, const int3
 cgp_status cgp_load_gr(const char* path, cgp_host_csr** out);
 cgp_status cgp_load_matrix_market(const char* path, cgp_host_csr** out);
 cgp_status cgp_load_csr_directory(const char* path, cgp_host_csr** out);
+/* The three loaders with flags (the ones above pass 0).
+ *   CGP_LOAD_UNIT_WEIGHTS: ignore stored weights (unit-weight SSSP); the reference's SSSP
+ *     driver reads .gr graphs this way (cgp/examples/algorithms/sssp/cgp_sssp.cu:53-56).
+ *   CGP_LOAD_ROUND_WEIGHTS: Matrix Market values that are not integers in [0, 2^32-1]
+ *     are rounded into that range; without it they are rejected (CGP_EIO), since
+ *     distances are exact uint32 sums. */
+#define CGP_FORMAT_GR 0
+#define CGP_FORMAT_MATRIX_MARKET 1
+#define CGP_FORMAT_CSR_DIRECTORY 2
+#define CGP_LOAD_UNIT_WEIGHTS 1
+#define CGP_LOAD_ROUND_WEIGHTS 2
+cgp_status cgp_load_graph(const char* path, int32_t format, int32_t flags, cgp_host_csr** out);
 /* Borrow the arrays of a host CSR (valid until cgp_host_csr_destroy). weights may be NULL. */
 cgp_status cgp_host_csr_view(const cgp_host_csr* g, int32_t* n_vertices, int64_t* n_edges,
                              const int32_t** row_offsets, const int32_t** col_indices,
@@ -289,11 +301,36 @@ typedef struct {
   int64_t next_bfs_pairs;    /* BFS (vertex, query) pairs of the next frontier */
   double advance_ms;         /* device-clock duration of the advance launch */
   double filter_ms;          /* ... of the compaction launch */
+  int64_t next_pairs;        /* (vertex, query) pairs of the next frontier, every kind */
+  int64_t next_vertices;     /* distinct vertices of the next frontier */
+  int64_t next_query_edges;  /* sum over the next frontier's pairs of the vertex degree */
+  int64_t edges_scanned;     /* profile bit 1: edges this iteration's advance scanned (else 0) */
+  double dense_ms;           /* device-clock duration of the lane-per-query launch (0: none) */
+  double end_ms;             /* device clock at this iteration's end, from the pass's start */
+  uint64_t frontier_union[8];/* OR of the frontier words this iteration's advance consumed:
+                                bit b of word w = slot 64w+b of the pass had frontier work */
 } cgp_iteration_record;
 
 /* Copies up to cap records of the last batch run; *n = records copied
  * (cap == 0: *n = records available). */
 cgp_status cgp_last_run_report(const cgp_engine* e, cgp_iteration_record* out, int32_t cap, int32_t* n);
+
+/* The last batch run's report as text, the fields of the reference driver's exports
+ * (cgp/examples/algorithms/bfs/cgp_bfs.cu:359-555: export_iter_profile CSV rows and
+ * export_json, with cgp_bfs::result_t / level_profile_t fields, cgp_bfs.hxx:107-149),
+ * filled from the device-written iteration records:
+ *   CGP_REPORT_JSON  one JSON object: graph and device, the queries (source, kind,
+ *                    completion level and time), per-level frontier sizes, unique
+ *                    frontier sizes, level edge counts (frontier degree sum, scanned
+ *                    edges, query-weighted), level modes and wall times, level profiles;
+ *   CGP_REPORT_CSV   the per-iteration profile rows (header line first).
+ * `dataset` names the graph in the output (may be NULL).  Writes at most cap bytes
+ * including the terminating NUL; *len = bytes the whole text needs (without the NUL),
+ * so a call with cap == 0 sizes the buffer.  CGP_EINVAL when no batch ran yet. */
+#define CGP_REPORT_JSON 0
+#define CGP_REPORT_CSV 1
+cgp_status cgp_last_run_report_text(const cgp_engine* e, int32_t format, const char* dataset, char* buf,
+                                    uint64_t cap, uint64_t* len);
 
 /* Per-query edge counts m_q = sum of out-degrees of the vertices reached by query q
  * (out[q*V+v] != CGP_INF); device-resident results in, n counts out (host). */

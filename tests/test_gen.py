@@ -199,3 +199,31 @@ def test_generator_argument_errors():
         HostCSR.rmat(8, 4, wmin=5, wmax=2)
     with pytest.raises(CgpError):
         HostCSR.grid(0, 5)
+
+
+def test_matrix_market_weights_are_exact_or_rejected(tmp_path):
+    # Distances are exact uint32 sums: a fractional, negative or too large
+    # value is an error unless the caller asks for rounding or unit weights.
+    p = tmp_path / "frac.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n3 3 2\n1 2 1.5\n2 3 2\n")
+    with pytest.raises(cgp_amd.CgpError) as ei:
+        HostCSR.load_matrix_market(p)
+    assert ei.value.status == cgp_amd.CGP_EIO and "not an integer" in str(ei.value)
+    g = HostCSR.load_matrix_market(p, round_weights=True)
+    assert g.weights.tolist() == [2, 2]
+    g = HostCSR.load_matrix_market(p, unit_weights=True)
+    assert g.weights is None and g.indices.tolist() == [1, 2]
+    n = tmp_path / "neg.mtx"
+    n.write_text("%%MatrixMarket matrix coordinate integer general\n2 2 1\n1 2 -3\n")
+    with pytest.raises(cgp_amd.CgpError):
+        HostCSR.load_matrix_market(n)
+    assert HostCSR.load_matrix_market(n, round_weights=True).weights.tolist() == [0]
+
+
+def test_gr_unit_weight_mode(tmp_path):
+    h = HostCSR.rmat(8, 8, seed=9, wmin=1, wmax=50)
+    p = tmp_path / "w.gr"
+    write_gr(p, h.offsets, h.indices, h.weights)
+    assert np.array_equal(HostCSR.load_gr(p).weights, h.weights)
+    u = HostCSR.load_gr(p, unit_weights=True)
+    assert u.weights is None and np.array_equal(u.indices, h.indices)
