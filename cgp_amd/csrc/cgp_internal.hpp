@@ -75,6 +75,9 @@ HostCSR load_matrix_market(const std::string& path, int32_t flags = 0);
 HostCSR load_csr_directory(const std::string& path, int32_t flags = 0);
 std::vector<int32_t> sample_sources(const HostCSR& g, int32_t n, uint64_t seed);
 bool is_symmetric(const HostCSR& g);
+// Every edge u -> v of weight w has a reverse edge v -> u of weight w
+// (structure symmetric and weighted; checked per edge).
+bool weights_symmetric(const HostCSR& g);
 
 }  // namespace cgpi
 

@@ -319,12 +319,14 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, 
                                            u32 thr, int rshift, u64 (&bm)[W], u64 (&sm)[W],
                                            u64 (&far)[W], bool& any_w, Tally& t) {
   const Cfg* cfg = P.cfg;
+  // In an SSSP pull iteration every vertex gathers its SSSP values itself.
+  const bool sssp_on = P.ctl->sssp_pull == 0;
   any_w = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const u64 f = P.front[(size_t)u * W + w];
     bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
-    u64 s = f & cfg->val_bits[w];
+    u64 s = sssp_on ? (f & cfg->val_bits[w]) : 0ull;
     if (__popcll(s) >= dense_t) s = 0;  // handled lane-per-query
     far[w] = 0;
     if (s && thr != INF) {
@@ -1223,7 +1225,7 @@ __device__ __forceinline__ void flush_tally(const KP& P, const Tally& t) {
 template <int W, int BS, bool SPARSE, int U, int TAB>
 __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   Ctl* C = P.ctl;
-  if (C->finished || (C->dense_mode == 0) != SPARSE) return;
+  if (C->finished || (C->dense_mode == 0) != SPARSE || C->sssp_pull) return;
   const u32 iter = C->iter, p = iter & 1u;
   const u32 nD = C->bcount[p][3];
   if (nD == 0) return;
@@ -1274,7 +1276,7 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
   Ctl* C = P.ctl;
-  if (C->finished) return;
+  if (C->finished || C->sssp_pull) return;
   const u32 iter = C->iter, p = iter & 1u;
   const u32 nN = C->bcount[p][4];
   if (nN == 0) return;
@@ -1306,6 +1308,132 @@ __global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
     const u64 now = (u64)wall_clock64();
     atomicMax(&C->adv_t1, now);
     atomicMax(&C->den_t1, now);
+  }
+}
+
+// SSSP pull (min-plus gather), the direction-optimising step of value-
+// carrying queries: every vertex v takes, for each SSSP query of each word,
+// the minimum over its edges (v, u) of value(u) + w(u, v) and lowers its own
+// value to it (pulling needs the reverse of every edge with the same weight,
+// which the host checks before choosing pull).  Items are (vertex, PCHUNK-edge
+// chunk) pairs of every vertex with edges, taken 16 at a time; a wave runs
+// one item per word with lane q as query q: 64 edge ids and weights are
+// loaded at once, then U neighbour rows per step are in flight.  Improved
+// pairs are published by atomicMin (a vertex's chunks run on different
+// waves) and their next bits by one atomicOr per (item, word).
+template <int W, bool SPARSE, int U>
+__device__ __forceinline__ void pull_sssp_item(const KP& P, u64 item, int rshift, u32 stamp, Tally& t) {
+  const int lane = lane_id();
+  const int R = 1 << rshift;
+  const int32_t v = (int32_t)(item >> 32);
+  const int32_t e0 = P.off[v] + (int32_t)((u32)item * PCHUNK);
+  const int32_t e1 = min(P.off[v + 1], e0 + PCHUNK);
+  const int lq = lane & (R - 1);
+  const u32 lq4 = (u32)lq * 4u;
+  ++t.pverts;
+  if (lane == 0) t.pedges += (u32)(e1 - e0);
+  bool claimed = false;
+  for (int w = 0; w < W; ++w) {
+    const u64 vb = P.cfg->val_bits[w];
+    if (!vb) continue;
+    const bool inw = lane < R && ((vb >> lq) & 1ull) != 0;
+    const u32 off = inw ? lq4 : 0u;  // other lanes re-read the row's first value (same lines)
+    u32 acc = INF;
+    for (int32_t eb = e0; eb < e1; eb += 64) {
+      const int32_t e = eb + lane;
+      const bool ok = e < e1;
+      const int32_t myu = ok ? P.idx[e] : 0;
+      const u32 myw = ok ? (P.wt ? P.wt[e] : 1u) : INF;  // steps past the end add INF
+      const int cnt = min(64, e1 - eb);
+      t.pwords += lane < cnt ? 1u : 0u;
+      for (int k0 = 0; k0 < cnt; k0 += U) {
+        u32 val[U], wk[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const int j = min(k0 + k, 63);
+          const u32 uk = (u32)__builtin_amdgcn_readlane(myu, j);
+          wk[k] = (u32)__builtin_amdgcn_readlane((int)myw, j);
+          val[k] = *(const gu32*)((u64)P.vals + (((u64)((u32)w * (u32)P.V + uk) << (rshift + 2)) + off));
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) acc = min(acc, __builtin_elementwise_add_sat(val[k], wk[k]));
+      }
+    }
+    const size_t vr = vrow(P.V, w, v, rshift);
+    const u32 cur = P.vals[vr + (inw ? lq : 0)];
+    const bool imp = inw && acc < cur;
+    const u64 m = __ballot(imp);
+    if (!m) continue;
+    if (imp) {
+      atomicMin(&P.vals[vr + lq], acc);
+      ++t.imp;
+    }
+    if (lane == 0) {
+      ++t.nnew;
+      if (SPARSE) {
+        if (atomicOr(&P.next[nxt_at<W>(v, w)], m) == 0) claimed = true;
+      } else {
+        atomicOr(&P.next[nxt_at<W>(v, w)], m);
+      }
+    }
+  }
+  if (SPARSE) {
+    const bool c = lane == 0 && claimed && claim_vertex(P.mark, v, stamp);
+    wave_append<int32_t>(c, v, &P.ctl->tcount, P.touched);
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_pull_sssp(KP P) {
+  Ctl* C = P.ctl;
+  if (C->finished || !C->sssp_pull) return;
+  if (threadIdx.x == 0) atomicMin(&C->adv_t0, (u64)wall_clock64());
+  const u32 iter = C->iter, p = iter & 1u;
+  const u32 stamp = C->stamp_base + iter + 1;
+  const int rshift = P.cfg->rshift;
+  const bool sparse = C->dense_mode == 0;
+  const u64 n = (u64)P.n_pull_items;
+  Tally t;
+  constexpr u32 FETCH_P = 16;
+  while (true) {
+    const u64 i0 = wave_fetch(&C->fetch[p][6], FETCH_P);
+    if (i0 >= n) break;
+    const u64 i1 = min(n, i0 + FETCH_P);
+    for (u64 i = i0; i < i1; ++i) {
+      if (sparse) pull_sssp_item<W, true, 8>(P, P.pull_items[i], rshift, stamp, t);
+      else pull_sssp_item<W, false, 8>(P, P.pull_items[i], rshift, stamp, t);
+    }
+  }
+  if (P.profile) {
+    prof_add(P, PF_PULL_VERTS, t.pverts);
+    prof_add(P, PF_PULL_EDGES, t.pedges);
+    prof_add(P, PF_PULL_WORDS, t.pwords);
+    prof_add(P, PF_IMPROVED, t.imp);
+    prof_add(P, PF_NEW_WORDS, t.nnew);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(&C->adv_t1, (u64)wall_clock64());
+}
+
+// (vertex << 32 | chunk) items of every vertex with edges, PCHUNK edges each
+// (order immaterial: an item is one vertex's gather over a range of edges).
+__global__ __launch_bounds__(BLOCK) void k_pull_items(const int32_t* off, int32_t V, u64* items, u32* count) {
+  for (int64_t b = (int64_t)blockIdx.x * BLOCK; b < V; b += (int64_t)gridDim.x * BLOCK) {
+    const int64_t v = b + threadIdx.x;
+    const int32_t d = v < V ? off[v + 1] - off[v] : 0;
+    const u32 nch = (u32)((d + PCHUNK - 1) / PCHUNK);
+    // Wave-level reservation of the wave's items, then each lane writes its own.
+    u32 x = nch;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const u32 y = (u32)__shfl_up((int)x, s, 64);
+      if (lane_id() >= s) x += y;
+    }
+    const u32 total = (u32)__shfl((int)x, 63, 64);
+    u32 base = 0;
+    if (lane_id() == 0 && total) base = atomicAdd(count, total);
+    base = (u32)__shfl((int)base, 0, 64) + x - nch;
+    for (u32 c = 0; c < nch; ++c) items[base + c] = ((u64)(u32)v << 32) | c;
   }
 }
 
@@ -1619,7 +1747,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3] + C->bcount[p][4];
     C->tcount = 0;
     C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = C->bcount[p][4] = 0;
-    C->fetch[p][0] = C->fetch[p][1] = C->fetch[p][2] = C->fetch[p][3] = C->fetch[p][4] = C->fetch[p][5] = 0;
+    for (int k = 0; k < 7; ++k) C->fetch[p][k] = 0;
     C->bfs_pairs[p] = 0;
     C->bfs_edges[p] = 0;
     C->done_blocks = 0;
@@ -1633,6 +1761,22 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       }
     }
     C->bfs_pull = pull;
+    // SSSP pull for the next iteration, on the same ratios applied to the
+    // SSSP (vertex, query) pairs of the next frontier (every pair that is
+    // not BFS: WCC queries have no frontier) and their degree-weighted count.
+    const u32 cur_spull = C->sssp_pull;
+    u32 spull = 0;
+    {
+      const u64 sp = napairs - fp, se = naqedges - fe;
+      if (sp > 0) {
+        if (cfg->sssp_pull_policy == 1) spull = 1;
+        else if (cfg->sssp_pull_policy == 2) {
+          if (cur_spull) spull = sp >= cfg->spush_pairs_thr ? 1u : 0u;
+          else spull = (sp >= cfg->spull_pairs_thr || se >= cfg->spull_edges_thr) ? 1u : 0u;
+        }
+      }
+    }
+    C->sssp_pull = spull;
     // Near-far: when no SSSP pair below the threshold is left, move it to
     // the smallest pending value plus delta (skipping empty buckets).
     if (cfg->delta != 0) {
@@ -1660,7 +1804,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       if (P.iters) {
         IterRec& r = P.iters[iter % ITER_RING];
         r.iter = iter;
-        r.pull = cur_pull;
+        r.pull = cur_pull | (cur_spull << 1);
         r.dense = scan ? 1u : 0u;
         r.frontier_items = cur_items;
         r.touched = n;
@@ -1695,7 +1839,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
     if (total == 0 || (C->max_iter != 0 && next_iter >= C->max_iter)) {
       C->finished = 1;
     } else {
-      if (pull) ++C->pull_iters;
+      if (pull || spull) ++C->pull_iters;
       else ++C->push_iters;
       if (C->dense_mode) ++C->dense_iters;
     }
@@ -1787,7 +1931,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_rows(const int32_t* off, int32_t
 // decrease somewhere, bit 1 a column index is out of [0, V), bit 2 some row
 // is not sorted, bit 3 some edge has no reverse edge (only tested on sorted
 // rows: a binary search in the target's row).
-enum : u32 { CSR_BAD_OFFSETS = 1, CSR_BAD_INDEX = 2, CSR_UNSORTED = 4, CSR_ASYMMETRIC = 8 };
+enum : u32 { CSR_BAD_OFFSETS = 1, CSR_BAD_INDEX = 2, CSR_UNSORTED = 4, CSR_ASYMMETRIC = 8, CSR_ASYM_WEIGHTS = 16 };
 
 __global__ __launch_bounds__(BLOCK) void k_csr_check_offsets(const int32_t* off, int32_t V, u32* flags) {
   bool bad = false;
@@ -1813,10 +1957,12 @@ __global__ __launch_bounds__(BLOCK) void k_csr_check_edges(const int32_t* rows, 
   }
 }
 
-// Every edge u -> v has v -> u (rows sorted, indices in range: checked before).
+// Every edge u -> v has v -> u (rows sorted, indices in range: checked
+// before), and with weights, one of the same weight.
 __global__ __launch_bounds__(BLOCK) void k_csr_check_symmetric(const int32_t* off, const int32_t* rows,
-                                                               const int32_t* idx, int64_t E, u32* flags) {
-  bool miss = false;
+                                                               const int32_t* idx, const u32* wt, int64_t E,
+                                                               u32* flags) {
+  bool miss = false, wmiss = false;
   for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * BLOCK) {
     const int32_t u = rows[e], v = idx[e];
     int32_t lo = off[v], hi = off[v + 1];
@@ -1825,9 +1971,16 @@ __global__ __launch_bounds__(BLOCK) void k_csr_check_symmetric(const int32_t* of
       if (idx[mid] < u) lo = mid + 1;
       else hi = mid;
     }
-    miss |= !(lo < off[v + 1] && idx[lo] == u);
+    const bool has = lo < off[v + 1] && idx[lo] == u;
+    miss |= !has;
+    if (has && wt) {
+      bool same = false;
+      for (int32_t x = lo; x < off[v + 1] && idx[x] == u && !same; ++x) same = wt[x] == wt[e];
+      wmiss |= !same;
+    }
   }
   if (__ballot(miss) && lane_id() == 0) atomicOr(flags, (u32)CSR_ASYMMETRIC);
+  if (__ballot(wmiss) && lane_id() == 0) atomicOr(flags, (u32)CSR_ASYM_WEIGHTS);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_sum_u32(const u32* x, int64_t n, u64* sum) {
@@ -1989,7 +2142,8 @@ void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, int tab) {
 
 // Optional advance launches of the current slot configuration: bit 0, the
 // lane-per-query kernels (some word has SSSP slots); bit 1, the narrow
-// kernel (some word is narrow).  The filter routes no item to a kernel that
+// kernel (some word is narrow); bit 2, the SSSP pull kernel (SSSP pull
+// steps enabled).  The filter routes no item to a kernel that
 // is not launched, and a BFS-only batch saves three launches per iteration.
 int launch_set(const cgp_engine* en) {
   const Cfg& c = *en->h_cfg;
@@ -1997,6 +2151,7 @@ int launch_set(const cgp_engine* en) {
   for (int w = 0; w < MAXW; ++w)
     if (c.val_bits[w]) ls |= 1;
   if (c.narrow_words) ls |= 2;
+  if (c.sssp_pull_policy) ls |= 4;
   return ls;
 }
 
@@ -2041,6 +2196,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
     hipLaunchKernelGGL((k_adv_push<W, 256>), grid, dim3(256), 0, sp, P);
   }
   const int ls = launch_set(en);
+  if (ls & 4) hipLaunchKernelGGL((k_pull_sssp<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (ls & 1) launch_dense<W>(s, en->advance_blocks(), P, en->opts.dense_unroll, dense_tab_mode(en));
   if (ls & 2) hipLaunchKernelGGL((k_adv_narrow<W>), dim3(en->advance_blocks()), dim3(BLOCK), 0, s, P);
   if (en->opts.concurrent_advance) {
@@ -2111,6 +2267,28 @@ u32 narrow_mask(const Cfg& cfg, int W) {
   }
   return m;
 }
+
+}  // namespace cgpi
+
+// SSSP pull items of the engine's graph: (vertex, PCHUNK-edge chunk) of every
+// vertex with edges, built once on the device.
+void cgp_engine::ensure_pull_items() {
+  if (pull_items.p) return;
+  const size_t cap = (size_t)g->V + (size_t)g->E / PCHUNK + 1;
+  pull_items.ensure(cap * 8 + 8);
+  u32* d_count = (u32*)((u64*)pull_items.p + cap);
+  HIP_CHECK(hipMemsetAsync(d_count, 0, 4, s));
+  hipLaunchKernelGGL(k_pull_items, dim3(cus * 8), dim3(BLOCK), 0, s, (const int32_t*)g->off, g->V,
+                     (u64*)pull_items.p, d_count);
+  HIP_CHECK(hipGetLastError());
+  u32 count = 0;
+  HIP_CHECK(hipMemcpyAsync(&count, d_count, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  n_pull_items = count;
+  drop_graphs();
+}
+
+namespace cgpi {
 
 void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
   if (n == 0) return;
@@ -2345,6 +2523,19 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     const bool can_pull = g->symmetric && nb > 0 && en->opts.traversal_mode != CGP_PUSH &&
                           (en->opts.traversal_mode == CGP_PULL || nb <= en->opts.pull_max_queries);
     cfg.pull_policy = !can_pull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
+    // SSSP pull steps gather over each vertex's own edges, so they need the
+    // reverse of every edge with the same weight; they are taken with
+    // postponing off (a gather relaxes every pair, near or far), and not in
+    // allocation-free runs (their item list is built on first use).
+    const int nsssp = ns - nb;
+    const bool can_spull = g->sym_weights && nsssp > 0 && cfg.delta == 0 && !no_alloc &&
+                           en->opts.traversal_mode != CGP_PUSH &&
+                           (en->opts.traversal_mode == CGP_PULL || nsssp <= en->opts.pull_max_queries);
+    cfg.sssp_pull_policy = !can_spull ? 0 : en->opts.traversal_mode == CGP_PULL ? 1 : 2;
+    cfg.spull_pairs_thr = std::max<u64>(1, (u64)(en->opts.pull_frontier_ratio * (double)nsssp * (double)V));
+    cfg.spull_edges_thr = std::max<u64>(1, (u64)(en->opts.pull_edge_ratio * (double)nsssp * (double)g->E));
+    cfg.spush_pairs_thr = cfg.spull_pairs_thr / 4 + 1;
+    if (cfg.sssp_pull_policy) en->ensure_pull_items();
     cfg.dense_fedges_thr = std::max<u64>(1, (u64)V / (u64)en->opts.frontier_dense_div);
     cfg.pull_pairs_thr = (u64)(en->opts.pull_frontier_ratio * (double)nb * (double)V);
     cfg.pull_edges_thr = (u64)(en->opts.pull_edge_ratio * (double)nb * (double)g->E);
@@ -2675,6 +2866,7 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
     g->E = E;
     g->weighted = wt != nullptr;
     g->symmetric = cgpi::is_symmetric(h);
+    g->sym_weights = g->symmetric && cgpi::weights_symmetric(h);
     if (wt && E > 0) {
       double sum = 0;
       for (int64_t e = 0; e < E; ++e) sum += wt[e];
@@ -2775,9 +2967,11 @@ cgp_status cgp_graph_create_csr_device(int32_t V, int64_t E, const int32_t* off,
     // Symmetry is only recognised on sorted rows (otherwise the graph is
     // treated as directed: every edge hooks in WCC, and BFS never pulls).
     if (!(f & CSR_UNSORTED) && E > 0)
-      hipLaunchKernelGGL(k_csr_check_symmetric, gb, dim3(BLOCK), 0, 0, off, (const int32_t*)g->rows, idx, E, flags);
+      hipLaunchKernelGGL(k_csr_check_symmetric, gb, dim3(BLOCK), 0, 0, off, (const int32_t*)g->rows, idx, wt, E,
+                         flags);
     f = read_flags();
     g->symmetric = !(f & (CSR_UNSORTED | CSR_ASYMMETRIC));
+    g->sym_weights = g->symmetric && !(f & CSR_ASYM_WEIGHTS);
     if (wt && E > 0) {
       hipLaunchKernelGGL(k_sum_u32, gb, dim3(BLOCK), 0, 0, wt, E, wsum);
       u64 sum = 0;
@@ -2951,7 +3145,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   DevBuf* bufs[] = {&en->visited, &en->front, &en->mark, &en->vals, &en->touched,
                     &en->small[0], &en->small[1], &en->medium[0], &en->medium[1], &en->large[0],
                     &en->large[1], &en->dense[0], &en->dense[1], &en->dense_n[0], &en->dense_n[1], &en->parent, &en->wcc_rows, &en->wcc_probe,
-                    &en->outbuf, &en->qmap, &en->slot_src, &en->iters};
+                    &en->outbuf, &en->qmap, &en->slot_src, &en->iters, &en->pull_items};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
   if (en->d_ctl) (void)hipFree(en->d_ctl);

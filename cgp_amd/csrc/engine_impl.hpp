@@ -38,6 +38,7 @@ struct cgp_graph {
   bool symmetric = false;
   bool weighted = false;
   bool borrowed = false;  // off / idx / wt belong to the caller (cgp_graph_create_csr_device)
+  bool sym_weights = false;  // symmetric, and every edge's reverse has the same weight (pull gathers)
   double mean_weight = 1.0;  // over all edges (1 when unweighted)
   int32_t* off = nullptr;
   int32_t* idx = nullptr;
@@ -126,6 +127,8 @@ struct cgp_engine {
   std::vector<PassSeed> rep_seed;
   double rep_wall_ms = 0;
   cgpi::DevBuf parent, wcc_rows, wcc_probe, outbuf, qmap, slot_src;
+  cgpi::DevBuf pull_items;  // SSSP pull items of the graph, built on first use
+  int64_t n_pull_items = 0;
   Ctl* d_ctl = nullptr;
   Cfg* d_cfg = nullptr;
   Ctl* h_ctl = nullptr;     // pinned
@@ -170,6 +173,8 @@ struct cgp_engine {
     P.pull_hi = g->pull_hi;
     P.n_pull_lo = g->n_pull_lo;
     P.n_pull_hi = g->n_pull_hi;
+    P.pull_items = (const u64*)pull_items.p;
+    P.n_pull_items = n_pull_items;
     // One buffer of per-vertex records: W visited words then W next words.
     P.visited = (u64*)visited.p;
     P.next = (u64*)visited.p;
@@ -243,6 +248,8 @@ struct cgp_engine {
     }
     if (moved) drop_graphs();
   }
+
+  void ensure_pull_items();  // engine.hip
 
   void ensure_vals(size_t elems) {
     if (vals.ensure(elems * 4)) drop_graphs();

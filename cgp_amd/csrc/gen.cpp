@@ -404,4 +404,30 @@ bool is_symmetric(const HostCSR& g) {
   return ok.load();
 }
 
+bool weights_symmetric(const HostCSR& g) {
+  if (g.wt.empty()) return true;
+  std::atomic<bool> ok{true};
+  parallel_for(g.V, [&](int64_t b, int64_t e, int) {
+    for (int64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
+      for (int32_t k = g.off[u]; k < g.off[u + 1]; ++k) {
+        const int32_t v = g.idx[k];
+        const uint32_t w = g.wt[k];
+        bool found = false;
+        const int32_t* r0 = g.idx.data() + g.off[v];
+        const int32_t* r1 = g.idx.data() + g.off[v + 1];
+        if (std::is_sorted(r0, r1)) {
+          for (const int32_t* x = std::lower_bound(r0, r1, (int32_t)u); x < r1 && *x == u && !found; ++x)
+            found = g.wt[(size_t)(x - g.idx.data())] == w;
+        } else {
+          for (const int32_t* x = r0; x < r1 && !found; ++x) found = *x == u && g.wt[(size_t)(x - g.idx.data())] == w;
+        }
+        if (!found) {
+          ok.store(false);
+          return;
+        }
+      }
+  });
+  return ok.load();
+}
+
 }  // namespace cgpi

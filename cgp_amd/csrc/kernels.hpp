@@ -50,6 +50,7 @@ constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
 constexpr int DCHUNK = 128;
 constexpr int NPROF = 18;
 constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
+constexpr int PCHUNK = 256;        // edges per SSSP pull item
 
 // Per-batch slot configuration (device resident, rewritten per pass).
 struct Cfg {
@@ -68,6 +69,11 @@ struct Cfg {
   u64 pull_edges_thr;   // ... or when their edge-weighted count >= this
   u64 push_pairs_thr;   // back to push when pairs fall below this
   u64 dense_fedges_thr; // dense frontier mode when the next frontier's edges reach this
+  int32_t sssp_pull_policy;  // SSSP words: 0 push only, 1 pull whenever SSSP bits are active,
+                             // 2 thresholds (min-plus gather over every vertex's edges)
+  u64 spull_pairs_thr;  // pull SSSP words when the frontier's SSSP (vertex, query) pairs reach this
+  u64 spull_edges_thr;  // ... or their degree-weighted count reaches this
+  u64 spush_pairs_thr;  // back to push when the pairs fall below this
 };
 
 // Device control block: counters and the iteration state machine.
@@ -76,6 +82,7 @@ struct Ctl {
   u32 stamp_base;   // mark stamp of iteration i is stamp_base + i + 1
   u32 finished;     // set when the next frontier is empty or max_iter reached
   u32 bfs_pull;     // BFS bits use the pull kernel this iteration
+  u32 sssp_pull;    // SSSP words are gathered (pulled) by every vertex this iteration
   u32 tcount;       // touched-list length
   u32 done_blocks;  // filter last-workgroup detection
   u32 max_iter;     // 0 = unlimited
@@ -90,7 +97,8 @@ struct Ctl {
   u32 bumps;        // threshold raises so far
   u32 pad;
   u32 bcount[2][5];           // [parity][small, medium, large items, dense items, narrow dense items]
-  u32 fetch[2][6];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high, N)
+  u32 fetch[2][7];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high, N,
+                              // SSSP pull items)
   u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
   u64 bfs_edges[2];           // ... weighted by degree
   u64 fedges[2];              // summed degree of the next frontier's vertices
@@ -115,7 +123,7 @@ struct Ctl {
 constexpr int ITER_RING = 65536;
 struct IterRec {
   u32 iter;
-  u32 pull;            // BFS bits went bottom-up this iteration
+  u32 pull;            // bit 0: BFS bits went bottom-up; bit 1: SSSP words were pulled
   u32 dense;           // dense frontier mode (fire-and-forget atomics, full filter scan)
   u32 frontier_items;  // work items of the frontier the advance consumed
   u32 touched;         // vertices the filter visited (V in dense mode)
@@ -144,6 +152,8 @@ struct KP {
   const int32_t* rows;
   const int32_t* pull_lo;   // vertices with 1 <= degree <= PULL_WAVE_DEG (lane per vertex)
   const int32_t* pull_hi;   // vertices with degree > PULL_WAVE_DEG (wave per vertex)
+  const u64* pull_items;    // SSSP pull: (vertex << 32 | PCHUNK-edge chunk) of every vertex with edges
+  int64_t n_pull_items;
   int32_t n_pull_lo;
   int32_t n_pull_hi;
   u64* visited;

@@ -96,8 +96,11 @@ typedef struct {
   int32_t concurrent_advance;  /* 1 = the push launch runs beside the lane-per-query launches on
                                   a second stream (fork/join events, also inside graphs); 0
                                   (default) = one stream */
-  int32_t pull_max_queries;    /* hybrid mode takes bottom-up steps only for batches of at most
-                                  this many BFS queries (0 = never; CGP_PULL always pulls) */
+  int32_t pull_max_queries;    /* hybrid mode takes pull steps only for batches of at most this
+                                  many queries of the kind: bottom-up steps for BFS, min-plus
+                                  gathers for SSSP (0 = never; CGP_PULL always pulls).  Pulls need
+                                  a symmetric graph (SSSP: with symmetric weights) and, for SSSP,
+                                  postponing off (sssp_delta 0) */
   int32_t advance_blocks;      /* workgroups of the advance launches (256-thread equivalents);
                                   0 = one per 8192 edges, between 1 and 8 per CU */
   int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
