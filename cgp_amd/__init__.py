@@ -393,6 +393,13 @@ class Engine:
         check(lib.cgp_serve_pending(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def serve_capacity(self):
+        """(slots, bytes of value rows) of the running session."""
+        n = C.c_int32(0)
+        b = C.c_uint64(0)
+        check(lib.cgp_serve_capacity(self._h, C.byref(n), C.byref(b)))
+        return n.value, b.value
+
     def serve_stop(self):
         check(lib.cgp_serve_stop(self._h))
 

@@ -187,6 +187,7 @@ SIGNATURES = {
     "cgp_serve_step": (_st, [_p, C.c_int32, C.POINTER(cgp_completion), C.c_int32, _i32p]),
     "cgp_serve_clock": (C.c_double, [_p]),
     "cgp_serve_pending": (_st, [_p, _i32p, _i32p]),
+    "cgp_serve_capacity": (_st, [_p, _i32p, _u64p]),
     "cgp_serve_release_result": (None, [_u32p]),
     "cgp_serve_stop": (_st, [_p]),
 }

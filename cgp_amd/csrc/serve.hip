@@ -12,7 +12,9 @@
 //     iteration), so a query admitted at iteration L gets the same depths as
 //     alone -- the reference writes the global level (hybrid_engine.hxx:172-175);
 //   * admission is capped by an HBM budget for the per-query value rows
-//     (V * 4 bytes each): arrivals queue, they never over-allocate;
+//     (V * 4 bytes each): the session allocates at most that many rows (the
+//     largest slot count of whole 64-slot words, or of one word of 2^k-lane
+//     rows, within the budget) and arrivals beyond its slots queue;
 //   * WCC requests are source-free: they share one hooking/pointer-jumping
 //     result per session and complete at admission.
 #include <deque>
@@ -69,7 +71,8 @@ struct SlotState {
 
 struct ServeState {
   int W = 1;               // mask words (rounded)
-  int nslots = 64;         // W * 64 slots exist
+  int rshift = 6;          // log2 of the value-row lanes (< 6: one word of 2^rshift slots)
+  int nslots = 64;         // W * 64 slots exist (2^rshift when rshift < 6)
   int budget_slots = 64;   // slots the HBM budget allows to be busy at once
   bool keep = true;
   std::chrono::steady_clock::time_point t0;
@@ -210,7 +213,8 @@ void admit(cgp_engine* en, ServeState* ss) {
     any = true;
   }
   if (!any) return;
-  cfg.narrow_words = cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, ss->W) : 0u;
+  cfg.narrow_words =
+      ss->rshift >= 4 && cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, ss->W) : 0u;
   HIP_CHECK(hipMemcpyAsync(en->d_cfg, en->h_cfg, sizeof(Cfg), hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipMemcpyAsync(ss->d_src.p, h_src, (size_t)ss->nslots * 4, hipMemcpyHostToDevice, en->s));
   HIP_CHECK(hipEventRecord(ss->staged, en->s));
@@ -276,7 +280,7 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
       a.clear[slot >> 6] |= 1ull << (slot & 63);
     }
     hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
-                       (u64*)en->visited.p, V, ss->W, 6, a, staging ? (u32*)staging->p : nullptr);
+                       (u64*)en->visited.p, V, ss->W, ss->rshift, a, staging ? (u32*)staging->p : nullptr);
     HIP_CHECK(hipGetLastError());
   }
   std::vector<cgp_completion> comps((size_t)k);
@@ -369,30 +373,47 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     const int32_t V = en->g->V;
     if (V <= 0) fail(CGP_EINVAL, "cgp_serve_start: empty graph");
     const uint64_t per_query = (uint64_t)V * 4;
-    int budget = (int)std::min<uint64_t>((uint64_t)MAXW * 64, en->opts.hbm_budget_bytes / per_query);
+    const int budget = (int)std::min<uint64_t>((uint64_t)MAXW * 64, en->opts.hbm_budget_bytes / per_query);
     if (budget < 1)
       fail(CGP_ECAPACITY, "cgp_serve_start: HBM budget smaller than one query's value row (" +
                               std::to_string(per_query) + " bytes)");
-    int slots = so && so->max_slots > 0 ? std::min(so->max_slots, MAXW * 64) : budget;
-    budget = std::min(budget, slots);
-    const int W = round_words((slots + 63) / 64);
+    const int want = so && so->max_slots > 0 ? std::min(so->max_slots, MAXW * 64) : budget;
+    // Slots the session allocates value rows for, never more than the
+    // budget: whole words of 64-lane rows, or one word of 2^k-lane rows.
+    const int cap = std::min(want, budget);
+    int W = 1;
+    int rshift = 6;
+    if (cap >= 64) {
+      while (W * 2 <= MAXW && W * 2 * 64 <= cap) W *= 2;
+    } else {
+      rshift = 0;
+      while ((2 << rshift) <= cap) ++rshift;
+    }
     std::unique_ptr<ServeState> ss(new ServeState());
     ss->W = W;
-    ss->nslots = W * 64;
-    ss->budget_slots = budget;
+    ss->rshift = rshift;
+    ss->nslots = rshift == 6 ? W * 64 : 1 << rshift;
+    ss->budget_slots = ss->nslots;
     ss->keep = !so || so->keep_results != 0;
     ss->slots.resize((size_t)ss->nslots);
     en->ensure_words(W);
-    en->ensure_vals((size_t)W * V * 64);
+    const size_t vals_elems = (size_t)W * V << rshift;
+    // The value buffer is exactly the session's rows (a larger one left by an
+    // earlier batch run is given back first, so the budget bounds it).
+    if (en->vals.bytes > vals_elems * 4) {
+      en->vals.release();
+      en->drop_graphs();
+    }
+    en->ensure_vals(vals_elems);
     en->stage((size_t)ss->nslots * 4 + 64);
     ss->d_src.ensure((size_t)ss->nslots * 4);
     // Clean slot state: every value row INF, no visited bits, no pending next bits.
-    fill_u32(en->s, en->vals.p, (size_t)W * V * 64, INF, en->cus);
+    fill_u32(en->s, en->vals.p, vals_elems, INF, en->cus);
     fill_u32(en->s, en->visited.p, (size_t)V * W * 4, 0u, en->cus);  // visited + next records
     en->next_dirty = false;
     Cfg& cfg = *en->h_cfg;
     std::memset(&cfg, 0, sizeof(Cfg));
-    cfg.rshift = 6;
+    cfg.rshift = rshift;
     cfg.W = W;
     cfg.pull_policy = 0;
     cfg.dense_t = en->opts.dense_threshold;
@@ -489,6 +510,14 @@ cgp_status cgp_serve_pending(const cgp_engine* en, int32_t* queued, int32_t* run
     if (!en || !en->serve) fail(CGP_EINVAL, "cgp_serve_pending: no serving session");
     if (queued) *queued = (int32_t)en->serve->queue.size();
     if (running) *running = en->serve->busy;
+  });
+}
+
+cgp_status cgp_serve_capacity(const cgp_engine* en, int32_t* slots, uint64_t* value_bytes) {
+  CGP_TRY({
+    if (!en || !en->serve) fail(CGP_EINVAL, "cgp_serve_capacity: no serving session");
+    if (slots) *slots = en->serve->nslots;
+    if (value_bytes) *value_bytes = (uint64_t)en->vals.bytes;
   });
 }
 

@@ -348,7 +348,8 @@ cgp_status cgp_engine_autotune(cgp_engine* e, const cgp_query* sample, int32_t n
 /* ---------------------------------------------------------------- serving (continuous batching) */
 /* Replaces run_replenish_pipeline (puercgp/include/puercgp/engine/replenish_engine.hxx:559-573). */
 typedef struct {
-  int32_t max_slots;        /* bitmask slots kept resident (multiple of 64); 0 = from budget */
+  int32_t max_slots;        /* bitmask slots kept resident; 0 = as many as the HBM budget allows
+                               (never more: see cgp_serve_capacity) */
   int32_t keep_results;     /* 1 = completions carry the V-length result row (host copy) */
 } cgp_serve_opts;
 
@@ -378,6 +379,11 @@ double cgp_serve_clock(const cgp_engine* e);
  * slot at once and completes when its result reaches host memory (the copy runs while
  * the traversal continues); cgp_serve_step on an idle engine finishes those copies. */
 cgp_status cgp_serve_pending(const cgp_engine* e, int32_t* queued, int32_t* running);
+/* Slots of the running session and the bytes of per-query value rows it holds, at most
+ * hbm_budget_bytes: the largest count of whole 64-slot words, or of one word of 2^k-lane
+ * rows, whose rows fit the budget (result rows staged for their copy to host memory are
+ * extra, one per completed query whose copy is in flight). */
+cgp_status cgp_serve_capacity(const cgp_engine* e, int32_t* slots, uint64_t* value_bytes);
 void cgp_serve_release_result(uint32_t* result);
 cgp_status cgp_serve_stop(cgp_engine* e);
 

@@ -123,6 +123,26 @@ def test_hbm_budget_caps_running_slots(gpu):
     en.serve_stop()
     for i in range(20):
         assert np.array_equal(done[i]["result"], O.sssp(og, src[i]))
+    # The session's value rows fit the budget, also after a larger batch run
+    # on the same engine, and for every budget size (2^k-lane rows below one
+    # word, whole 64-slot words above).
+    big = cgp_amd.Engine(g, hbm_budget_bytes=5 * g.V * 4)
+    big.run_batch([SSSP] * 20, [int(x) for x in src])
+    for rows, max_slots, slots in ((5, 0, 4), (3, 0, 2), (1, 0, 1), (100, 0, 64), (300, 0, 256), (300, 70, 64)):
+        e2 = big if rows == 5 else cgp_amd.Engine(g, hbm_budget_bytes=rows * g.V * 4)
+        e2.serve_start(max_slots=max_slots)
+        n, b = e2.serve_capacity()
+        assert n == slots and b <= rows * g.V * 4 and b == n * g.V * 4, (rows, max_slots, n, b)
+        for i in range(6):
+            e2.serve_submit(SSSP if i % 2 else BFS, int(src[i]), i, 0.0)
+        got = {}
+        while len(got) < 6:
+            for c in e2.serve_step(max_iterations=0):
+                got[c["ticket"]] = c
+        for i in range(6):
+            exp = O.sssp(og, src[i]) if i % 2 else O.bfs(og, src[i])
+            assert np.array_equal(got[i]["result"], exp), (rows, i)
+        e2.serve_stop()
     small = cgp_amd.Engine(g, hbm_budget_bytes=g.V * 4 - 1)
     with pytest.raises(cgp_amd.CgpError) as e:
         small.serve_start()
