@@ -97,7 +97,7 @@ __device__ __forceinline__ u32 wave_reserve(u32 cnt, u32* counter) {
 // BLOCK-thread workgroup must call it (uniform control flow); lds holds
 // (BLOCK/64 + 1) * N words.
 template <int N>
-__device__ __forceinline__ void block_reserve(const u32 (&cnt)[N], u32* counters, u32 (&at)[N], u32* lds) {
+__device__ __forceinline__ void block_reserve(const u32 (&cnt)[N], Ctr* counters, u32 (&at)[N], u32* lds) {
   constexpr int NW = BLOCK / 64;
   const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
   u32 incl[N];
@@ -118,7 +118,7 @@ __device__ __forceinline__ void block_reserve(const u32 (&cnt)[N], u32* counters
     u32 tot = 0;
 #pragma unroll
     for (int wv = 0; wv < NW; ++wv) tot += lds[wv * N + k];
-    lds[NW * N + k] = tot ? atomicAdd(&counters[k], tot) : 0u;
+    lds[NW * N + k] = tot ? atomicAdd(&counters[k].v, tot) : 0u;
   }
   __syncthreads();
 #pragma unroll
@@ -319,14 +319,12 @@ __device__ __forceinline__ void load_masks(const KP& P, int32_t u, bool bfs_on, 
                                            u32 thr, int rshift, u64 (&bm)[W], u64 (&sm)[W],
                                            u64 (&far)[W], bool& any_w, Tally& t) {
   const Cfg* cfg = P.cfg;
-  // In an SSSP pull iteration every vertex gathers its SSSP values itself.
-  const bool sssp_on = P.ctl->sssp_pull == 0;
   any_w = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const u64 f = P.front[(size_t)u * W + w];
     bm[w] = bfs_on ? (f & cfg->bfs_bits[w]) : 0ull;
-    u64 s = sssp_on ? (f & cfg->val_bits[w]) : 0ull;
+    u64 s = f & cfg->val_bits[w];
     if (__popcll(s) >= dense_t) s = 0;  // handled lane-per-query
     far[w] = 0;
     if (s && thr != INF) {
@@ -388,7 +386,7 @@ __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0,
   load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
   if (lead) {
     const bool c = postpone<W, SPARSE>(P, u, far, stamp, t);
-    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount, P.touched);
+    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount.v, P.touched);
   }
   bool any_b = false;
 #pragma unroll
@@ -408,7 +406,7 @@ __device__ __forceinline__ void expand_lanes(const KP& P, int32_t u, int32_t e0,
     }
     ++t.edges;
     const bool claimed = relax_edge<W, SPARSE>(P, u, v, wv, bm, sm, stamp, rshift, thr, t);
-    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
+    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount.v, P.touched);
   }
 }
 
@@ -457,7 +455,7 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
   }
   {
     const bool c = have && postpone<W, SPARSE>(P, u, far, stamp, t);
-    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount, P.touched);
+    if (SPARSE) wave_append<int32_t>(c, u, &P.ctl->tcount.v, P.touched);
   }
   const bool work = have && (any_b || any_s);
   const int32_t e0 = work ? P.off[u] : 0;
@@ -497,7 +495,7 @@ __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t*
       ++t.edges;
       claimed = relax_edge<W, SPARSE>(P, uj, v, wv, bj, sj, stamp, rshift, thr, t);
     }
-    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount, P.touched);
+    if (SPARSE) wave_append<int32_t>(claimed, v, &P.ctl->tcount.v, P.touched);
   }
 }
 
@@ -573,7 +571,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
               atomicOr(&P.next[nxt_at<W>(uk, wk)], fm);
             }
           }
-          if (SPARSE) wave_append<int32_t>(claimed, uk, &P.ctl->tcount, P.touched);
+          if (SPARSE) wave_append<int32_t>(claimed, uk, &P.ctl->tcount.v, P.touched);
         }
       }
     }
@@ -684,7 +682,7 @@ __device__ __forceinline__ void expand_flat_dense(const KP& P, u32 base, u32 n, 
             const int wk = __builtin_amdgcn_readlane(myword, k0 + k);
             if (atomicOr(&P.next[nxt_at<W>(vk, wk)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
           }
-          wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
+          wave_append<int32_t>(claimed, vk, &P.ctl->tcount.v, P.touched);
         }
       }
       // Dense mode: lane k0+k owns edge k0+k's next-word update, one masked
@@ -854,7 +852,7 @@ __device__ __forceinline__ void expand_flat_dense_tab(const KP& P, u32 base, u32
                 ++t.nnew;
                 if (atomicOr(&P.next[nxt_at<W>(vk, wp)], m) == 0) claimed = claim_vertex(P.mark, vk, stamp);
               }
-              wave_append<int32_t>(claimed, vk, &P.ctl->tcount, P.touched);
+              wave_append<int32_t>(claimed, vk, &P.ctl->tcount.v, P.touched);
             }
           }
         }
@@ -977,7 +975,7 @@ __device__ __forceinline__ void expand_flat_narrow(const KP& P, u32 base, u32 n,
             ++t.nnew;
             if (atomicOr(&P.next[nxt_at<W>(myv, myword)], (u64)mg) == 0) claimed = claim_vertex(P.mark, myv, stamp);
           }
-          wave_append<int32_t>(claimed, myv, &P.ctl->tcount, P.touched);
+          wave_append<int32_t>(claimed, myv, &P.ctl->tcount.v, P.touched);
         }
       }
     };
@@ -1103,6 +1101,33 @@ __device__ __forceinline__ u32 wave_fetch(u32* ctr, u32 n) {
   return (u32)__shfl((int)i, 0, 64);
 }
 
+// Wave-level claims of `chunk` units from NSHARD contiguous shards of [0, n):
+// shard s covers [n*s/NSHARD, n*(s+1)/NSHARD) and has cursor cur[s] (units
+// handed out so far).  A wave starts at its workgroup's shard and moves to the
+// next when that one is exhausted (cursors only grow, so an exhausted shard
+// stays exhausted); after NSHARD exhausted shards every unit has been claimed.
+// Claims never cross a shard's end.  All state is wave-uniform.
+struct ShardFetch {
+  int s, left;
+  __device__ __forceinline__ ShardFetch() : s((int)(blockIdx.x % NSHARD)), left(NSHARD) {}
+  __device__ __forceinline__ bool next(Ctr* cur, u32 n, u32 chunk, u32& lo, u32& hi) {
+    while (left > 0) {
+      const u32 b = (u32)(((u64)n * (u32)s) / NSHARD), e = (u32)(((u64)n * (u32)(s + 1)) / NSHARD);
+      if (b < e) {
+        const u32 i = wave_fetch(&cur[s].v, chunk);
+        if (i < e - b) {
+          lo = b + i;
+          hi = min(e, lo + chunk);
+          return true;
+        }
+      }
+      s = s + 1 == NSHARD ? 0 : s + 1;
+      --left;
+    }
+    return false;
+  }
+};
+
 // Body of the advance for one frontier mode; workgroups pull work
 // dynamically, heaviest first:
 //   L  one workgroup per LCHUNK-edge chunk of a high-degree vertex
@@ -1118,13 +1143,15 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
   const bool bfs_on = C->bfs_pull == 0;
   const u32 stamp = C->stamp_base + iter + 1;
   const u32 thr = C->thr;
-  const int rshift = P.cfg->rshift, dense_t = P.cfg->dense_t;
+  // In an SSSP pull iteration every vertex gathers its SSSP values itself: a
+  // threshold of 0 leaves no SSSP word to the push phases.
+  const int rshift = P.cfg->rshift, dense_t = C->sssp_pull ? 0 : P.cfg->dense_t;
   const int lane = lane_id();
   // Phase L.
-  const u32 nL = C->bcount[p][2];
+  const u32 nL = C->bcount[p][2].v;
   while (nL) {
     __syncthreads();
-    if (threadIdx.x == 0) *s_item = atomicAdd(&C->fetch[p][0], 1u);
+    if (threadIdx.x == 0) *s_item = atomicAdd(&C->fetch[p][0].v, 1u);
     __syncthreads();
     const u32 i = *s_item;
     if (i >= nL) break;
@@ -1140,24 +1167,20 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
   // 64 small vertices).
   constexpr u32 FETCH_M = 4, FETCH_S = 4;
   // Phase M.
-  const u32 nM = C->bcount[p][1];
-  while (nM) {
-    const u32 i0 = wave_fetch(&C->fetch[p][2], FETCH_M);
-    if (i0 >= nM) break;
-    const u32 i1 = min(nM, i0 + FETCH_M);
-    for (u32 i = i0; i < i1; ++i) {
+  const u32 nM = C->bcount[p][1].v;
+  u32 lo, hi;
+  for (ShardFetch sf; nM && sf.next(C->shard[p][SH_M], nM, FETCH_M, lo, hi);) {
+    for (u32 i = lo; i < hi; ++i) {
       const int32_t u = P.medium[p][i];
       const int32_t e0 = P.off[u], e1 = P.off[u + 1];
       expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
     }
   }
   // Phase S.
-  const u32 nS = C->bcount[p][0];
-  while (nS) {
-    const u32 b0 = wave_fetch(&C->fetch[p][3], 64u * FETCH_S);
-    if (b0 >= nS) break;
-    for (u32 base = b0; base < nS && base < b0 + 64u * FETCH_S; base += 64u)
-      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, nS, bfs_on, dense_t, rshift, stamp, thr, t);
+  const u32 nS = C->bcount[p][0].v;
+  for (ShardFetch sf; nS && sf.next(C->shard[p][SH_S], nS, 64u * FETCH_S, lo, hi);) {
+    for (u32 base = lo; base < hi; base += 64u)
+      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, hi, bfs_on, dense_t, rshift, stamp, thr, t);
   }
   // Phase P: high-degree vertices one wave each (64 neighbours per step,
   // early exit on a wave-wide OR), then the rest one lane each.
@@ -1167,19 +1190,19 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
     for (int w = 0; w < W; ++w) bb[w] = P.cfg->bfs_bits[w];
     const u32 nH = (u32)P.n_pull_hi;
     while (nH) {
-      const u32 i = wave_fetch(&C->fetch[p][4], 1u);
+      const u32 i = wave_fetch(&C->fetch[p][4].v, 1u);
       if (i >= nH) break;
       const int32_t v = P.pull_hi[i];
       bool claimed = false;
       if (pull_vertex_wave<W, SPARSE>(P, v, bb, stamp, t) && lane == 0) claimed = true;
-      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount.v, P.touched);
     }
     const int64_t nLo = P.n_pull_lo;
     for (int64_t base = (int64_t)blockIdx.x * BS; base < nLo; base += (int64_t)gridDim.x * BS) {
       const int64_t i = base + threadIdx.x;
       const int32_t v = i < nLo ? P.pull_lo[i] : 0;
       const bool claimed = i < nLo && pull_vertex<W, SPARSE>(P, v, bb, stamp, t);
-      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+      if (SPARSE) wave_append<int32_t>(claimed, v, &C->tcount.v, P.touched);
     }
   }
 }
@@ -1227,7 +1250,7 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   Ctl* C = P.ctl;
   if (C->finished || (C->dense_mode == 0) != SPARSE || C->sssp_pull) return;
   const u32 iter = C->iter, p = iter & 1u;
-  const u32 nD = C->bcount[p][3];
+  const u32 nD = C->bcount[p][3].v;
   if (nD == 0) return;
   if (threadIdx.x == 0) {
     const u64 now = (u64)wall_clock64();
@@ -1246,9 +1269,12 @@ __global__ __launch_bounds__(BS) void k_adv_dense(KP P) {
   uint4* tab = s_tab + (TAB ? (threadIdx.x & ~63u) : 0u);
   u32* wtab = s_wt + (TAB == 2 ? (threadIdx.x & ~63u) : 0u);
   // One group of 64 items per fetch: items carry up to DCHUNK edges each, so
-  // coarser claims leave a long tail (measured 2.5x slower at 4 groups).
+  // coarser claims leave a long tail (measured 2.5x slower at 4 groups).  A
+  // single cursor: the per-XCD shards of the other phases measured about
+  // 1 ms slower here on the RMAT-22 mixed batch (29.8 against 28.9 ms), the
+  // stealing at the end of the shards lengthening the tail.
   while (true) {
-    const u32 i = wave_fetch(&C->fetch[p][1], 64u);
+    const u32 i = wave_fetch(&C->fetch[p][1].v, 64u);
     if (i >= nD) break;
     if (TAB) expand_flat_dense_tab<W, SPARSE, U, TAB == 2>(P, i, nD, rshift, stamp, tab, wtab, t);
     else if (thr == INF) expand_flat_dense<W, SPARSE, U, false>(P, i, nD, rshift, stamp, thr, t);
@@ -1278,7 +1304,7 @@ __global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
   Ctl* C = P.ctl;
   if (C->finished || C->sssp_pull) return;
   const u32 iter = C->iter, p = iter & 1u;
-  const u32 nN = C->bcount[p][4];
+  const u32 nN = C->bcount[p][4].v;
   if (nN == 0) return;
   if (threadIdx.x == 0) {
     const u64 now = (u64)wall_clock64();
@@ -1289,11 +1315,10 @@ __global__ __launch_bounds__(BLOCK) void k_adv_narrow(KP P) {
   const int rshift = P.cfg->rshift;
   const bool sparse = C->dense_mode == 0;
   Tally t;
-  while (true) {
-    const u32 i = wave_fetch(&C->fetch[p][5], 64u);
-    if (i >= nN) break;
-    if (sparse) expand_flat_narrow<W, true>(P, i, nN, rshift, stamp, t);
-    else expand_flat_narrow<W, false>(P, i, nN, rshift, stamp, t);
+  u32 lo, hi;
+  for (ShardFetch sf; sf.next(C->shard[p][SH_N], nN, 64u, lo, hi);) {
+    if (sparse) expand_flat_narrow<W, true>(P, lo, hi, rshift, stamp, t);
+    else expand_flat_narrow<W, false>(P, lo, hi, rshift, stamp, t);
   }
   if (P.profile) {
     prof_add(P, PF_DENSE_ITEMS, t.ditems);
@@ -1379,7 +1404,7 @@ __device__ __forceinline__ void pull_sssp_item(const KP& P, u64 item, int rshift
   }
   if (SPARSE) {
     const bool c = lane == 0 && claimed && claim_vertex(P.mark, v, stamp);
-    wave_append<int32_t>(c, v, &P.ctl->tcount, P.touched);
+    wave_append<int32_t>(c, v, &P.ctl->tcount.v, P.touched);
   }
 }
 
@@ -1392,14 +1417,12 @@ __global__ __launch_bounds__(BLOCK) void k_pull_sssp(KP P) {
   const u32 stamp = C->stamp_base + iter + 1;
   const int rshift = P.cfg->rshift;
   const bool sparse = C->dense_mode == 0;
-  const u64 n = (u64)P.n_pull_items;
+  const u32 n = (u32)P.n_pull_items;
   Tally t;
   constexpr u32 FETCH_P = 16;
-  while (true) {
-    const u64 i0 = wave_fetch(&C->fetch[p][6], FETCH_P);
-    if (i0 >= n) break;
-    const u64 i1 = min(n, i0 + FETCH_P);
-    for (u64 i = i0; i < i1; ++i) {
+  u32 lo, hi;
+  for (ShardFetch sf; sf.next(C->shard[p][SH_P], n, FETCH_P, lo, hi);) {
+    for (u32 i = lo; i < hi; ++i) {
       if (sparse) pull_sssp_item<W, true, 8>(P, P.pull_items[i], rshift, stamp, t);
       else pull_sssp_item<W, false, 8>(P, P.pull_items[i], rshift, stamp, t);
     }
@@ -1475,7 +1498,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed(KP P, const int32_t* slot_src, i
     }
     claimed = claim_vertex(P.mark, v, stamp);
   }
-  wave_append<int32_t>(claimed, v, &C->tcount, P.touched);
+  wave_append<int32_t>(claimed, v, &C->tcount.v, P.touched);
 }
 
 // ------------------------------------------------------------------ filter
@@ -1524,7 +1547,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   const Cfg* cfg = P.cfg;
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
   const bool scan = C->dense_mode != 0;
-  const u32 n = scan ? (u32)P.V : C->tcount;
+  const u32 n = scan ? (u32)P.V : C->tcount.v;
   const int rshift = cfg->rshift, dense_t = cfg->dense_t;
   const int lane = lane_id();
   u64 bb[W], vb[W], un[W];
@@ -1723,15 +1746,15 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    const u32 t = atomicAdd(&C->done_blocks, 1u);
+    const u32 t = atomicAdd(&C->done_blocks.v, 1u);
     s_last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (s_last && threadIdx.x == 0) {
     __threadfence();
-    const u32 total = atomicAdd(&C->bcount[np][0], 0u) + atomicAdd(&C->bcount[np][1], 0u) +
-                      atomicAdd(&C->bcount[np][2], 0u) + atomicAdd(&C->bcount[np][3], 0u) +
-                      atomicAdd(&C->bcount[np][4], 0u);
+    const u32 total = atomicAdd(&C->bcount[np][0].v, 0u) + atomicAdd(&C->bcount[np][1].v, 0u) +
+                      atomicAdd(&C->bcount[np][2].v, 0u) + atomicAdd(&C->bcount[np][3].v, 0u) +
+                      atomicAdd(&C->bcount[np][4].v, 0u);
     const u64 fp = atomicAdd(&C->bfs_pairs[np], 0ull), fe = atomicAdd(&C->bfs_edges[np], 0ull);
     const u64 fall = atomicAdd(&C->fedges[np], 0ull);
     const u64 napairs = atomicAdd(&C->npairs[np], 0ull), naverts = atomicAdd(&C->nverts[np], 0ull),
@@ -1744,13 +1767,15 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       consumed[w] = C->last_union[w];
       C->last_union[w] = atomicAnd(&C->union_[w], 0ull);
     }
-    const u32 cur_items = C->bcount[p][0] + C->bcount[p][1] + C->bcount[p][2] + C->bcount[p][3] + C->bcount[p][4];
-    C->tcount = 0;
-    C->bcount[p][0] = C->bcount[p][1] = C->bcount[p][2] = C->bcount[p][3] = C->bcount[p][4] = 0;
-    for (int k = 0; k < 7; ++k) C->fetch[p][k] = 0;
+    const u32 cur_items = C->bcount[p][0].v + C->bcount[p][1].v + C->bcount[p][2].v + C->bcount[p][3].v + C->bcount[p][4].v;
+    C->tcount.v = 0;
+    C->bcount[p][0].v = C->bcount[p][1].v = C->bcount[p][2].v = C->bcount[p][3].v = C->bcount[p][4].v = 0;
+    for (int k = 0; k < 7; ++k) C->fetch[p][k].v = 0;
+    for (int k = 0; k < 5; ++k)
+      for (int s = 0; s < NSHARD; ++s) C->shard[p][k][s].v = 0;
     C->bfs_pairs[p] = 0;
     C->bfs_edges[p] = 0;
-    C->done_blocks = 0;
+    C->done_blocks.v = 0;
     const u32 cur_pull = C->bfs_pull;
     u32 pull = 0;
     if (fp > 0) {
@@ -2377,7 +2402,7 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     } else {
       dispatch_iteration(en, W, P);
     }
-    HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
+    HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, CTL_HOST_BYTES, hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
     const u32 after = en->h_ctl->iter;
     if (en->h_ctl->finished) break;

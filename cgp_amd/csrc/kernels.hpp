@@ -32,6 +32,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace cgpk {
@@ -51,6 +52,13 @@ constexpr int DCHUNK = 128;
 constexpr int NPROF = 18;
 constexpr int PULL_WAVE_DEG = 64;  // pull: vertices above this degree get a whole wave
 constexpr int PCHUNK = 256;        // edges per SSSP pull item
+// Work lists are split into NSHARD contiguous shards, one per XCD of the
+// MI355X (workgroups are placed on the XCDs round-robin), each with its own
+// cursor: a wave takes work from its own XCD's shard first, so the shard's
+// CSR rows and source rows stay in that XCD's L2, and the cursors' atomics
+// spread over NSHARD lines instead of one.
+constexpr int NSHARD = 8;
+enum : int { SH_M = 0, SH_S, SH_D, SH_N, SH_P };
 
 // Per-batch slot configuration (device resident, rewritten per pass).
 struct Cfg {
@@ -76,6 +84,15 @@ struct Cfg {
   u64 spush_pairs_thr;  // back to push when the pairs fall below this
 };
 
+// A device counter on a 256-byte line of its own: the work cursors and
+// reservation counters that every wave or workgroup hits with atomics share
+// no line with each other or with the fields every kernel reads (atomics on
+// one line serialise in one L2 channel, and reads of that line queue behind
+// them).
+struct alignas(256) Ctr {
+  u32 v;
+};
+
 // Device control block: counters and the iteration state machine.
 struct Ctl {
   u32 iter;         // iteration of the current run (0xFFFFFFFF during seeding)
@@ -83,8 +100,6 @@ struct Ctl {
   u32 finished;     // set when the next frontier is empty or max_iter reached
   u32 bfs_pull;     // BFS bits use the pull kernel this iteration
   u32 sssp_pull;    // SSSP words are gathered (pulled) by every vertex this iteration
-  u32 tcount;       // touched-list length
-  u32 done_blocks;  // filter last-workgroup detection
   u32 max_iter;     // 0 = unlimited
   u32 push_iters;
   u32 pull_iters;
@@ -96,9 +111,6 @@ struct Ctl {
   u32 near_new;     // nonzero when this iteration produced an SSSP pair below thr
   u32 bumps;        // threshold raises so far
   u32 pad;
-  u32 bcount[2][5];           // [parity][small, medium, large items, dense items, narrow dense items]
-  u32 fetch[2][7];            // [parity] dynamic work cursors of the advance (L, D, M, S, pull-high, N,
-                              // SSSP pull items)
   u64 bfs_pairs[2];           // BFS frontier (vertex, query) pairs, by parity
   u64 bfs_edges[2];           // ... weighted by degree
   u64 fedges[2];              // summed degree of the next frontier's vertices
@@ -116,7 +128,17 @@ struct Ctl {
   u64 npairs[2], nverts[2], nqedges[2];
   u64 seed_pairs, seed_verts, seed_edges, seed_qedges;
   u64 t_start;  // device clock at the end of the seeding filter
+  // Counters hit by atomics from every wave or workgroup, each on its own
+  // line (the host never reads them: it copies back only the fields above).
+  Ctr tcount;                 // touched-list length
+  Ctr done_blocks;            // filter last-workgroup detection
+  Ctr bcount[2][5];           // [parity][small, medium, large items, dense items, narrow dense items]
+  Ctr fetch[2][7];            // [parity] work cursors of the advance without shards (L, D, unused,
+                              // unused, pull-high, unused, unused)
+  Ctr shard[2][5][NSHARD];    // [parity][M, S, unused, N, SSSP pull] per-shard work cursors
 };
+// Bytes of Ctl the host reads back after a run of iterations.
+constexpr size_t CTL_HOST_BYTES = offsetof(Ctl, tcount);
 
 // Per-iteration record, written by the filter's last workgroup into a ring
 // of ITER_RING entries (index = iteration % ITER_RING), no host sync.

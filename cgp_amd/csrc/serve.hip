@@ -472,7 +472,7 @@ cgp_status cgp_serve_step(cgp_engine* en, int32_t max_iterations, cgp_completion
       }
       if (max_iterations > 0 && ran >= max_iterations) break;
       dispatch_iteration(en, W, en->kp());
-      HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, en->s));
+      HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, CTL_HOST_BYTES, hipMemcpyDeviceToHost, en->s));
       HIP_CHECK(hipStreamSynchronize(en->s));
       ++ran;
       const Ctl& hc = *en->h_ctl;
