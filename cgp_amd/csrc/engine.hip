@@ -2380,9 +2380,9 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
   const KP P = en->kp();
   const bool use_graph = en->opts.use_hip_graph != 0 && !no_graph;
   const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
-  cgp_engine::GExec& gx = en->gx[wlog(W)];
   const int ls = launch_set(en);
-  if (use_graph && (!gx.exec || gx.K != K || gx.prof != en->opts.profile || gx.ls != ls)) {
+  cgp_engine::GExec& gx = en->gx[wlog(W)][ls];
+  if (use_graph && (!gx.exec || gx.K != K || gx.prof != en->opts.profile)) {
     if (gx.exec) (void)hipGraphExecDestroy(gx.exec);
     gx.exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -2394,7 +2394,6 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     HIP_CHECK(ie);
     gx.K = K;
     gx.prof = en->opts.profile;
-    gx.ls = ls;
   }
   for (;;) {
     if (use_graph) {
@@ -2611,7 +2610,10 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       report.resize(first + nrec);
       if (nrec) {
         std::vector<IterRec> ring(nrec);
-        HIP_CHECK(hipMemcpy(ring.data(), en->iters.p, nrec * sizeof(IterRec), hipMemcpyDeviceToHost));
+        // On the engine's stream: the legacy stream would wait on (and fail
+        // beside) other threads' engines capturing graphs.
+        HIP_CHECK(hipMemcpyAsync(ring.data(), en->iters.p, nrec * sizeof(IterRec), hipMemcpyDeviceToHost, en->s));
+        HIP_CHECK(hipStreamSynchronize(en->s));
         for (size_t k = 0; k < nrec; ++k) {
           const u32 it = hc.iter - (u32)nrec + (u32)k;
           report[first + k] = ring[it % ITER_RING];
@@ -2651,7 +2653,8 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   st.wall_ms = now_ms() - t0;
   en->rep_wall_ms = st.wall_ms;
   if (!out_dev) {
-    HIP_CHECK(hipMemcpy(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost, en->s));
+    HIP_CHECK(hipStreamSynchronize(en->s));
   }
   if (en->timed()) {
     for (auto& e : en->ev) {

@@ -139,14 +139,16 @@ struct cgp_engine {
   bool next_dirty = false;
   bool busy = false;
   std::mutex mu;
-  // Captured iteration graphs, one per mask-word width.
+  // Captured iteration graphs, one per mask-word width and set of optional
+  // launches (so batches of different kinds on one engine, as in one query
+  // at a time, reuse their graphs instead of capturing again).
+  static constexpr int NLS = 8;  // launch sets: lane-per-query | narrow << 1 | SSSP pull << 2
   struct GExec {
     hipGraphExec_t exec = nullptr;
     int K = 0;
     int prof = -1;
-    int ls = -1;  // optional launches captured (launch_set)
   };
-  GExec gx[4];
+  GExec gx[4][NLS];
   double clock_khz = 100000.0;  // device wall-clock rate (wall_clock64 ticks)
   // Per-phase timing of init / WCC / output (launch timing on, outside graphs).
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -199,10 +201,12 @@ struct cgp_engine {
 
   void drop_graphs() {
     for (int k = 0; k < 4; ++k) {
-      if (gx[k].exec) (void)hipGraphExecDestroy(gx[k].exec);
-      gx[k].exec = nullptr;
-      gx[k].K = 0;
-      gx[k].prof = -1;
+      for (GExec& x : gx[k]) {
+        if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        x.exec = nullptr;
+        x.K = 0;
+        x.prof = -1;
+      }
     }
   }
 
