@@ -1884,61 +1884,95 @@ __global__ void k_fill_u32(u32* p, size_t n, u32 val) {
   for (size_t k = n4 * 4 + i; k < n; k += stride) p[k] = val;
 }
 
-// Transposes the vertex-major value rows of word w into query-major output
-// rows: out[qmap[w*64+q] * V + v].  One workgroup per 64-vertex tile.
-// Transposes OUT_VT vertices x R slots of one word's value rows into the
-// query-major output (each output row segment is OUT_VT * 4 = 256 bytes;
-// 128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22; 16-byte global
-// accesses took 2.9 to 1.55 ms).
+// Fills the first 4 << l4 lanes of each of nrows 64-lane value rows with
+// val (16-byte stores).
+__global__ void k_fill_row_prefix(u32* p, size_t nrows, int l4, u32 val) {
+  const size_t n = nrows << l4;
+  const size_t stride = (size_t)gridDim.x * BLOCK;
+  uint4* p4 = reinterpret_cast<uint4*>(p);
+  const uint4 v4 = make_uint4(val, val, val, val);
+  const size_t cmask = ((size_t)1 << l4) - 1;
+  for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += stride)
+    p4[((k >> l4) << 4) + (k & cmask)] = v4;
+}
+
+// BFS slots of each word of a pass: their value-row entries are written only
+// where the slot's visited bit is set (the depth, by the compaction), so the
+// output takes CGP_INF wherever the bit is clear.
+struct OutMasks {
+  u64 bfs[MAXW];
+};
+
+// Transposes the vertex-major value rows of words 0..Wused-1 into query-major
+// output rows: out[qmap[w*64+q] * V + v], CGP_INF for a BFS slot whose
+// visited bit is clear.  One workgroup per 64-vertex tile (each output row
+// segment is OUT_VT * 4 = 256 bytes; 128-vertex tiles measured slower: 4.5
+// vs 2.9 ms on RMAT-22; 16-byte global accesses took 2.9 to 1.55 ms), the
+// words in turn, so the tile's visited records (2W words per vertex) are
+// read from HBM once.
 constexpr int OUT_VT = 64;
-__global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, int32_t V, int rshift,
-                                                  const int32_t* qmap, u32* out, int vec) {
+__global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* visited, int32_t V, int W,
+                                                  int Wused, int rshift, const int32_t* qmap, OutMasks m,
+                                                  u32* out, int vec) {
   __shared__ u32 tile[64][OUT_VT + 1];
-  const int w = blockIdx.y;
+  __shared__ u64 s_vis[OUT_VT];
   const int R = 1 << rshift;
   const int32_t v0 = (int32_t)blockIdx.x * OUT_VT;
   const int nv = min(OUT_VT, V - v0);
-  const size_t rowbase = vrow(V, w, v0, rshift);
-  if (vec) {
-    // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by the
-    // launcher): 16-byte loads of 4 slots of a vertex, 16-byte stores of 4
-    // vertices of a query.
-    const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
+  for (int w = 0; w < Wused; ++w) {
+    const u64 bm = m.bfs[w];
+    const size_t rowbase = vrow(V, w, v0, rshift);
+    if (bm && (int)threadIdx.x < nv) s_vis[threadIdx.x] = visited[(size_t)(v0 + (int)threadIdx.x) * (2 * W) + w];
+    if (vec) {
+      // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by
+      // the launcher): 16-byte loads of 4 slots of a vertex, 16-byte stores
+      // of 4 vertices of a query.
+      const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
 #pragma unroll
-    for (int pass = 0; pass < 4; ++pass) {
-      const int j = pass * 16 + sub;
-      if (j < nv) {
-        const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
-        tile[c4 + 0][j] = x.x;
-        tile[c4 + 1][j] = x.y;
-        tile[c4 + 2][j] = x.z;
-        tile[c4 + 3][j] = x.w;
+      for (int pass = 0; pass < 4; ++pass) {
+        const int j = pass * 16 + sub;
+        if (j < nv) {
+          const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
+          tile[c4 + 0][j] = x.x;
+          tile[c4 + 1][j] = x.y;
+          tile[c4 + 2][j] = x.z;
+          tile[c4 + 3][j] = x.w;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int pass = 0; pass < 4; ++pass) {
+        const int q = pass * 16 + sub;
+        const int32_t dst = qmap[w * 64 + q];
+        if (dst < 0) continue;
+        const bool bfs = (bm >> q) & 1ull;
+        u32 r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j = min(c4 + i, OUT_VT - 1);
+          r[i] = bfs && !((s_vis[j] >> q) & 1ull) ? INF : tile[q][j];
+        }
+        u32* o = out + (size_t)dst * V + v0 + c4;
+        if (c4 + 4 <= nv) {
+          *(uint4*)o = make_uint4(r[0], r[1], r[2], r[3]);
+        } else {
+          for (int i = 0; c4 + i < nv; ++i) o[i] = r[i];
+        }
+      }
+    } else {
+      for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
+        const int j = t >> rshift, q = t & (R - 1);
+        if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
+      }
+      __syncthreads();
+      for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
+        const int q = t / OUT_VT, j = t % OUT_VT;
+        const int32_t dst = qmap[w * 64 + q];
+        if (dst >= 0 && j < nv)
+          out[(size_t)dst * V + v0 + j] = ((bm >> q) & 1ull) && !((s_vis[j] >> q) & 1ull) ? INF : tile[q][j];
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int pass = 0; pass < 4; ++pass) {
-      const int q = pass * 16 + sub;
-      const int32_t dst = qmap[w * 64 + q];
-      if (dst < 0) continue;
-      u32* o = out + (size_t)dst * V + v0 + c4;
-      if (c4 + 4 <= nv) {
-        *(uint4*)o = make_uint4(tile[q][c4], tile[q][c4 + 1], tile[q][c4 + 2], tile[q][c4 + 3]);
-      } else {
-        for (int i = 0; c4 + i < nv; ++i) o[i] = tile[q][c4 + i];
-      }
-    }
-    return;
-  }
-  for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
-    const int j = t >> rshift, q = t & (R - 1);
-    if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
-    const int q = t / OUT_VT, j = t % OUT_VT;
-    const int32_t dst = qmap[w * 64 + q];
-    if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = tile[q][j];
+    __syncthreads();  // the tile and s_vis are rewritten for the next word
   }
 }
 
@@ -2322,6 +2356,13 @@ void fill_u32(hipStream_t s, void* p, size_t n, u32 v, int cus) {
   HIP_CHECK(hipGetLastError());
 }
 
+void fill_row_prefix(hipStream_t s, void* p, size_t nrows, int l4, u32 v, int cus) {
+  if (nrows == 0) return;
+  size_t blocks = std::min<size_t>(((nrows << l4) + BLOCK - 1) / BLOCK, (size_t)cus * 16);
+  hipLaunchKernelGGL(k_fill_row_prefix, dim3((unsigned)blocks), dim3(BLOCK), 0, s, (u32*)p, nrows, l4, v);
+  HIP_CHECK(hipGetLastError());
+}
+
 static int wlog(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
 
 void compute_wcc(cgp_engine* en) {
@@ -2588,7 +2629,18 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     HIP_CHECK(hipMemcpyAsync(en->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->slot_src.p, h_src, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
     HIP_CHECK(hipMemcpyAsync(en->qmap.p, h_qmap, (size_t)W * 64 * 4, hipMemcpyHostToDevice, en->s));
-    fill_u32(en->s, en->vals.p, (size_t)W * V * R, INF, en->cus);
+    // Value rows start at CGP_INF only where SSSP slots read them: words
+    // without SSSP slots are left as they are, and of a 64-slot word only the
+    // 16-, 32- or 64-lane prefix holding its SSSP slots (they are packed from
+    // lane 0) is filled.  BFS slots are masked by their visited bits at output.
+    for (int w = 0; w < W; ++w) {
+      const u64 vb = cfg.val_bits[w];
+      if (!vb) continue;
+      u32* wv = (u32*)en->vals.p + (size_t)w * V * R;
+      const int hi = 64 - __builtin_clzll(vb);
+      if (R < 64 || hi > 32) fill_u32(en->s, wv, (size_t)V * R, INF, en->cus);
+      else fill_row_prefix(en->s, wv, (size_t)V, hi > 16 ? 3 : 2, INF, en->cus);
+    }
     // Visited + next records (2W words per vertex) start clear; the layout
     // depends on W, so the whole region is cleared for every pass.
     fill_u32(en->s, en->visited.p, (size_t)V * W * 4, 0u, en->cus);
@@ -2632,8 +2684,10 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
     en->tick(CAT_OUT, true);
     const int vec = rshift == 6 && (V & 3) == 0 && ((uintptr_t)dout & 15) == 0;
-    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK), 0, en->s,
-                       (const u32*)en->vals.p, V, rshift, (const int32_t*)en->qmap.p, dout, vec);
+    OutMasks om{};
+    for (int w = 0; w < Wused; ++w) om.bfs[w] = cfg.bfs_bits[w];
+    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT), dim3(BLOCK), 0, en->s, (const u32*)en->vals.p,
+                       (const u64*)en->visited.p, V, W, Wused, rshift, (const int32_t*)en->qmap.p, om, dout, vec);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
   }

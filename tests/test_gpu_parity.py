@@ -317,6 +317,39 @@ def test_edge_cases(gpu):
         en.run_batch([7], [0])
 
 
+@pytest.mark.parametrize("n_sssp,n_bfs", [(64, 64), (64, 130), (80, 48), (8, 8), (3, 5)])
+def test_bfs_slots_over_stale_value_rows(gpu, n_sssp, n_bfs):
+    # Value rows of BFS slots are not reset between batches (only SSSP slots
+    # start at CGP_INF): a BFS row must read CGP_INF wherever its visited bit
+    # is clear, also over rows left finite by an earlier SSSP batch.  Two
+    # disjoint RMAT-10 components: SSSP queries in A first, then BFS queries
+    # in B on the same engine, whose rows must be CGP_INF on all of A.
+    a = cgp_amd.HostCSR.rmat(10, 8, seed=31, wmin=1, wmax=50)
+    b = cgp_amd.HostCSR.rmat(10, 8, seed=32, wmin=1, wmax=50)
+    ra = np.repeat(np.arange(a.V), np.diff(a.offsets))
+    rb = np.repeat(np.arange(b.V), np.diff(b.offsets))
+    src = np.concatenate([ra, rb + a.V]).astype(np.int32)
+    dst = np.concatenate([a.indices, b.indices + a.V]).astype(np.int32)
+    wt = np.concatenate([a.weights, b.weights]).astype(np.uint32)
+    h = cgp_amd.HostCSR.from_coo(a.V + b.V, src, dst, wt)
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    sa = [int(x) for x in a.sample_sources(n_sssp, seed=5)]
+    sb = [int(x) + a.V for x in b.sample_sources(n_bfs, seed=6)]
+    en = cgp_amd.Engine(g)
+    out1 = en.run_batch([SSSP] * n_sssp, sa)[0]
+    assert np.array_equal(out1, oracle_rows(og, [SSSP] * n_sssp, sa))
+    assert (out1[:, : a.V] != INF).any()
+    out2 = en.run_batch([BFS] * n_bfs, sb)[0]
+    assert np.all(out2[:, : a.V] == INF)
+    assert np.array_equal(out2, oracle_rows(og, [BFS] * n_bfs, sb))
+    # Mixed: SSSP slots of the new batch sit in B, BFS slots in A again.
+    kinds = [SSSP] * min(n_sssp, 16) + [BFS] * n_bfs
+    srcs = [int(x) + a.V for x in b.sample_sources(min(n_sssp, 16), seed=7)] + [int(x) for x in a.sample_sources(n_bfs, seed=8)]
+    out3 = en.run_batch(kinds, srcs)[0]
+    assert np.array_equal(out3, oracle_rows(og, kinds, srcs))
+
+
 def test_directed_graph_uses_push_and_weak_components(gpu):
     h = cgp_amd.HostCSR.from_coo(7, [0, 0, 2, 3, 5], [1, 2, 3, 4, 6], [1, 2, 3, 4, 5])
     g = cgp_amd.Graph.from_host(h)
