@@ -1631,8 +1631,21 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       cnt[4] += (u32)__popc(nmask) * dch;
       kdeg[k] = (u32)deg;
       kfl[k] = (need_bin ? 1u : 0u) | (b << 1) | (dmask << 4) | (nmask << 12);
-      // BFS depths: lanes of the wave write R-wide value rows, 64/R vertices per pass.
-      if (rshift == 6) {
+      // BFS depths.  Bit planes (batch runs): the iteration's depth is the
+      // same for every new bit, so each set bit b of it ORs the new bits
+      // into plane b (one 8-byte atomic per word with new bits, no return).
+      if (cfg->depth_planes) {
+        for (u32 bits = depth_base; bits; bits &= bits - 1) {
+          u64* const pl = cfg->plane[__builtin_ctz(bits)];
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (nb[w])
+              __hip_atomic_fetch_or(&pl[(size_t)w * (size_t)P.V + (size_t)v], nb[w], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
+      } else if (rshift == 6) {
+        // Value rows (serving, allocation-free runs): lanes of the wave write
+        // R-wide rows, 64/R vertices per pass.
         // 64-slot rows: one row store per vertex with new bits, its vertex
         // and new-bit mask read into scalar registers and the mask used as
         // the store's exec mask.
@@ -1901,42 +1914,68 @@ __global__ void k_fill_row_prefix(u32* p, size_t nrows, int l4, u32 val) {
 // output takes CGP_INF wherever the bit is clear.
 struct OutMasks {
   u64 bfs[MAXW];
+  u64 val[MAXW];
+  const u64* plane[MAXPLANES];  // BFS depth bit planes (n_planes of them; 0: depths in the value rows)
+  int n_planes;
 };
 
-// Transposes the vertex-major value rows of words 0..Wused-1 into query-major
-// output rows: out[qmap[w*64+q] * V + v], CGP_INF for a BFS slot whose
-// visited bit is clear.  One workgroup per 64-vertex tile (each output row
-// segment is OUT_VT * 4 = 256 bytes; 128-vertex tiles measured slower: 4.5
-// vs 2.9 ms on RMAT-22; 16-byte global accesses took 2.9 to 1.55 ms), the
-// words in turn, so the tile's visited records (2W words per vertex) are
-// read from HBM once.
+// Writes the query-major output rows of words 0..Wused-1: out[qmap[w*64+q] *
+// V + v].  SSSP slots take their value rows, transposed through an LDS tile.
+// BFS slots take CGP_INF wherever their visited bit is clear and otherwise
+// their depth: from the depth bit planes when there are any (only the
+// value-row lines holding SSSP slots are then read), else from the value
+// rows.  One workgroup per 64-vertex tile (each output row segment is
+// OUT_VT * 4 = 256 bytes; 128-vertex tiles measured slower: 4.5 vs 2.9 ms on
+// RMAT-22; 16-byte global accesses took 2.9 to 1.55 ms), the words in turn,
+// so the tile's visited records (2W words per vertex) are read from HBM once.
 constexpr int OUT_VT = 64;
 __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* visited, int32_t V, int W,
                                                   int Wused, int rshift, const int32_t* qmap, OutMasks m,
                                                   u32* out, int vec) {
   __shared__ u32 tile[64][OUT_VT + 1];
   __shared__ u64 s_vis[OUT_VT];
+  extern __shared__ u64 s_pl[];  // [n_planes][OUT_VT], sized at launch
   const int R = 1 << rshift;
   const int32_t v0 = (int32_t)blockIdx.x * OUT_VT;
   const int nv = min(OUT_VT, V - v0);
+  const int np = m.n_planes;
   for (int w = 0; w < Wused; ++w) {
     const u64 bm = m.bfs[w];
+    const u64 rd = np ? m.val[w] : (m.val[w] | bm);  // slots whose value rows are read
     const size_t rowbase = vrow(V, w, v0, rshift);
-    if (bm && (int)threadIdx.x < nv) s_vis[threadIdx.x] = visited[(size_t)(v0 + (int)threadIdx.x) * (2 * W) + w];
+    if (bm) {
+      for (int t = threadIdx.x; t < (np + 1) * OUT_VT; t += BLOCK) {
+        const int b = t / OUT_VT, j = t % OUT_VT;
+        if (j >= nv) continue;
+        if (b == 0) s_vis[j] = visited[(size_t)(v0 + j) * (2 * W) + w];
+        else s_pl[(b - 1) * OUT_VT + j] = m.plane[b - 1][(size_t)w * (size_t)V + (size_t)(v0 + j)];
+      }
+    }
+    // Value of BFS slot q at tile vertex j (the tile's rows when there are
+    // no planes).
+    auto bfs_val = [&](int q, int j) -> u32 {
+      if (!((s_vis[j] >> q) & 1ull)) return INF;
+      if (!np) return tile[q][j];
+      u32 d = 0;
+      for (int b = 0; b < np; ++b) d |= (u32)((s_pl[b * OUT_VT + j] >> q) & 1ull) << b;
+      return d;
+    };
     if (vec) {
       // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by
       // the launcher): 16-byte loads of 4 slots of a vertex, 16-byte stores
       // of 4 vertices of a query.
       const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
+      if ((rd >> c4) & 0xFull) {
 #pragma unroll
-      for (int pass = 0; pass < 4; ++pass) {
-        const int j = pass * 16 + sub;
-        if (j < nv) {
-          const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
-          tile[c4 + 0][j] = x.x;
-          tile[c4 + 1][j] = x.y;
-          tile[c4 + 2][j] = x.z;
-          tile[c4 + 3][j] = x.w;
+        for (int pass = 0; pass < 4; ++pass) {
+          const int j = pass * 16 + sub;
+          if (j < nv) {
+            const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
+            tile[c4 + 0][j] = x.x;
+            tile[c4 + 1][j] = x.y;
+            tile[c4 + 2][j] = x.z;
+            tile[c4 + 3][j] = x.w;
+          }
         }
       }
       __syncthreads();
@@ -1950,7 +1989,7 @@ __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* vi
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int j = min(c4 + i, OUT_VT - 1);
-          r[i] = bfs && !((s_vis[j] >> q) & 1ull) ? INF : tile[q][j];
+          r[i] = bfs ? bfs_val(q, j) : tile[q][j];
         }
         u32* o = out + (size_t)dst * V + v0 + c4;
         if (c4 + 4 <= nv) {
@@ -1962,17 +2001,16 @@ __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* vi
     } else {
       for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
         const int j = t >> rshift, q = t & (R - 1);
-        if (j < nv) tile[q][j] = vals[rowbase + (size_t)t];
+        if (j < nv && ((rd >> q) & 1ull)) tile[q][j] = vals[rowbase + (size_t)t];
       }
       __syncthreads();
       for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
         const int q = t / OUT_VT, j = t % OUT_VT;
         const int32_t dst = qmap[w * 64 + q];
-        if (dst >= 0 && j < nv)
-          out[(size_t)dst * V + v0 + j] = ((bm >> q) & 1ull) && !((s_vis[j] >> q) & 1ull) ? INF : tile[q][j];
+        if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = ((bm >> q) & 1ull) ? bfs_val(q, j) : tile[q][j];
       }
     }
-    __syncthreads();  // the tile and s_vis are rewritten for the next word
+    __syncthreads();  // the tile, s_vis and s_pl are rewritten for the next word
   }
 }
 
@@ -2417,6 +2455,29 @@ void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) 
 // Runs the frontier iterations of one pass to convergence: replays of a
 // captured K-iteration graph (or direct launches), one host readback of the
 // control block per replay.
+// Depth bit planes [plane_count, need) of the running pass: allocated (a
+// word of V bits per mask word), zeroed over the pass's BFS words, and
+// published in the slot configuration (on the device too when `live`: the
+// pass is running and its configuration was uploaded).
+static void add_depth_planes(cgp_engine* en, int W, int need, bool live) {
+  const size_t V = (size_t)en->g->V;
+  Cfg& cfg = *en->h_cfg;
+  need = std::min(need, MAXPLANES);
+  for (int b = en->plane_count; b < need; ++b) {
+    en->plane[b].ensure(V * (size_t)W * 8);
+    u64* pl = (u64*)en->plane[b].p;
+    for (int w = 0; w < W; ++w)
+      if ((en->plane_words >> w) & 1u) fill_u32(en->s, pl + (size_t)w * V, V * 2, 0u, en->cus);
+    cfg.plane[b] = pl;
+    if (live)
+      HIP_CHECK(hipMemcpyAsync((char*)en->d_cfg + offsetof(Cfg, plane) + (size_t)b * sizeof(u64*), &cfg.plane[b],
+                               sizeof(u64*), hipMemcpyHostToDevice, en->s));
+  }
+  en->plane_count = std::max(en->plane_count, need);
+}
+
+static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
 static void run_iterations(cgp_engine* en, int W, bool no_graph) {
   const KP P = en->kp();
   const bool use_graph = en->opts.use_hip_graph != 0 && !no_graph;
@@ -2436,7 +2497,11 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     gx.K = K;
     gx.prof = en->opts.profile;
   }
+  u32 iter = 0;  // the seeding compaction leaves iteration 0
   for (;;) {
+    // Iteration i writes BFS depth i + 1: planes for every depth of the
+    // iterations this launch may run.
+    if (en->h_cfg->depth_planes) add_depth_planes(en, W, bit_length((u64)iter + (u64)K), true);
     if (use_graph) {
       HIP_CHECK(hipGraphLaunch(gx.exec, en->s));
     } else {
@@ -2445,6 +2510,7 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, CTL_HOST_BYTES, hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
     const u32 after = en->h_ctl->iter;
+    iter = after;
     if (en->h_ctl->finished) break;
     // Between two raises of the near-far threshold the iterations are a
     // Bellman-Ford frontier over the pairs below it, bounded by V + 1 (each
@@ -2607,6 +2673,16 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     cfg.push_pairs_thr = cfg.pull_pairs_thr / 4 + 1;
     if (cfg.pull_pairs_thr == 0) cfg.pull_pairs_thr = 1;
     if (cfg.pull_edges_thr == 0) cfg.pull_edges_thr = 1;
+    // BFS depth bit planes: in batch runs every slot starts at iteration 0
+    // (allocation-free runs keep depths in the value rows).
+    en->plane_count = 0;
+    en->plane_words = 0;
+    if (!no_alloc && nb > 0) {
+      cfg.depth_planes = 1;
+      for (int w = 0; w < W; ++w)
+        if (cfg.bfs_bits[w]) en->plane_words |= 1u << w;
+      add_depth_planes(en, W, 4, false);  // depths below 16
+    }
     // Control block.
     Ctl& c = *en->h_ctl;
     std::memset(&c, 0, sizeof(Ctl));
@@ -2685,8 +2761,14 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->tick(CAT_OUT, true);
     const int vec = rshift == 6 && (V & 3) == 0 && ((uintptr_t)dout & 15) == 0;
     OutMasks om{};
-    for (int w = 0; w < Wused; ++w) om.bfs[w] = cfg.bfs_bits[w];
-    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT), dim3(BLOCK), 0, en->s, (const u32*)en->vals.p,
+    for (int w = 0; w < Wused; ++w) {
+      om.bfs[w] = cfg.bfs_bits[w];
+      om.val[w] = cfg.val_bits[w];
+    }
+    om.n_planes = cfg.depth_planes ? en->plane_count : 0;
+    for (int b = 0; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
+    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT), dim3(BLOCK),
+                       (size_t)om.n_planes * OUT_VT * sizeof(u64), en->s, (const u32*)en->vals.p,
                        (const u64*)en->visited.p, V, W, Wused, rshift, (const int32_t*)en->qmap.p, om, dout, vec);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
@@ -3229,6 +3311,7 @@ void cgp_engine_destroy(cgp_engine* en) {
                     &en->large[1], &en->dense[0], &en->dense[1], &en->dense_n[0], &en->dense_n[1], &en->parent, &en->wcc_rows, &en->wcc_probe,
                     &en->outbuf, &en->qmap, &en->slot_src, &en->iters, &en->pull_items};
   for (DevBuf* b : bufs) b->release();
+  for (DevBuf& b : en->plane) b.release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
   if (en->d_ctl) (void)hipFree(en->d_ctl);
   if (en->d_cfg) (void)hipFree(en->d_cfg);

@@ -127,6 +127,11 @@ struct cgp_engine {
   std::vector<PassSeed> rep_seed;
   double rep_wall_ms = 0;
   cgpi::DevBuf parent, wcc_rows, wcc_probe, outbuf, qmap, slot_src;
+  // BFS depth bit planes of the running pass (batch runs): plane_count
+  // planes zeroed over the pass's BFS words (plane_words bit w).
+  cgpi::DevBuf plane[cgpk::MAXPLANES];
+  int plane_count = 0;
+  u32 plane_words = 0;
   cgpi::DevBuf pull_items;  // SSSP pull items of the graph, built on first use
   int64_t n_pull_items = 0;
   Ctl* d_ctl = nullptr;

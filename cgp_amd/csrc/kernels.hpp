@@ -41,6 +41,7 @@ using u32 = uint32_t;
 using u64 = unsigned long long;
 
 constexpr int MAXW = 8;          // mask words per pass (512 slots)
+constexpr int MAXPLANES = 32;    // BFS depth bit planes (depths below 2^32)
 constexpr u32 INF = 0xFFFFFFFFu;
 constexpr int BLOCK = 256;       // workgroup size of every kernel (4 waves)
 constexpr int LCHUNK = 4096;     // edges per large-bin workgroup item
@@ -82,6 +83,14 @@ struct Cfg {
   u64 spull_pairs_thr;  // pull SSSP words when the frontier's SSSP (vertex, query) pairs reach this
   u64 spull_edges_thr;  // ... or their degree-weighted count reaches this
   u64 spush_pairs_thr;  // back to push when the pairs fall below this
+  // Batch runs (every slot starts at iteration 0, so a compaction's new BFS
+  // bits all get the same depth): depths are kept as bit planes, bit q of
+  // plane[b][w * V + v] = bit b of the depth of slot 64w+q at vertex v,
+  // written with one OR per set bit of the depth and per word with new bits.
+  // Planes below plane_count exist and start at zero; more are added by the
+  // host before the depths need them.  0: depths go into the value rows.
+  int32_t depth_planes;
+  u64* plane[MAXPLANES];
 };
 
 // A device counter on a 256-byte line of its own: the work cursors and
