@@ -187,19 +187,22 @@ def cpu_baseline(h, kinds, sources, m_all, seconds_target):
 
 
 # ---------------------------------------------------------------- rocprofv3 traffic
-def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R, runs, launches_per_run):
+CALIBRATION_BYTES = 1 << 30  # the plain copy each counter pass ends with
+
+
+def pmc_traffic(pmc_dir, kernel_prefix, runs, launches_per_run):
     """HBM bytes per working launch of `kernel_prefix` from rocprofv3
     FETCH_SIZE / WRITE_SIZE CSVs (KiB per dispatch): each counter's total over
     the profiled run, scaled by the factor that makes the same counter read
-    the known bytes of k_output (reads the words*V*R value rows, writes
-    out_rows*V*4 bytes), divided by the working launches of that run (`runs`
-    batch runs of `launches_per_run` launches that did work each, the same
-    launches the algorithmic bytes are averaged over)."""
+    the known bytes of a plain copy kernel (k_copy16: reads and writes
+    CALIBRATION_BYTES once each), divided by the working launches of that run
+    (`runs` batch runs of `launches_per_run` launches that did work each, the
+    same launches the algorithmic bytes are averaged over)."""
     import csv
     import glob
     tot = 0.0
     factors = {}
-    for counter, known in (("FETCH_SIZE", 4.0 * words * V * R), ("WRITE_SIZE", 4.0 * out_rows * V)):
+    for counter, known in (("FETCH_SIZE", float(CALIBRATION_BYTES)), ("WRITE_SIZE", float(CALIBRATION_BYTES))):
         vals, cal = [], []
         for f in glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
             for r in csv.DictReader(open(f)):
@@ -207,7 +210,7 @@ def pmc_traffic(pmc_dir, kernel_prefix, out_rows, V, words, R, runs, launches_pe
                     continue
                 name = r.get("Kernel_Name", "")
                 v = float(r.get("Counter_Value", 0.0)) * 1024.0
-                if "k_output" in name:
+                if "k_copy16" in name:
                     cal.append(v)
                 elif kernel_prefix in name:
                     vals.append(v)
@@ -233,7 +236,7 @@ def run_pmc_children(workload):
         cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o",
                name, "--", sys.executable, os.path.join(ROOT, "bench.py"), "--workload", workload, "--steps", "1",
                "--warmup", "0", "--no-one-at-a-time", "--no-cpu", "--no-serve", "--no-pmc", "--no-baselines",
-               "--no-host"]
+               "--no-host", "--calibration-copy"]
         log("rocprofv3 %s pass" % counter)
         r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=sys.stderr)
         if r.returncode != 0:
@@ -504,6 +507,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end with one device-to-device copy of %d MiB through the library's copy kernel "
+                         "(the known traffic the counter passes are scaled by)" % (CALIBRATION_BYTES >> 20))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU (query-sharded multi-GPU is a design note, "
@@ -696,10 +702,8 @@ def main():
         except Exception as e:  # the traffic figure is reported, never required
             pmc_dir, pmc_note = None, repr(e)
     if pmc_dir:
-        R = 64 if (n_bfs + n_sssp) > 64 else 1 << max(0, (n_bfs + n_sssp - 1).bit_length())
         for k in kern:
-            tr, factors = pmc_traffic(pmc_dir, k + "<", n_bfs + n_sssp, V, W, R, args.pmc_runs,
-                                      kern[k][3] / max(1, args.steps))
+            tr, factors = pmc_traffic(pmc_dir, k + "<", args.pmc_runs, kern[k][3] / max(1, args.steps))
             if tr:
                 kr = kernels_report[k]
                 kr["traffic"] = tr
@@ -768,6 +772,10 @@ def main():
             result["latency_ms"]["serving_p99_at_95pct_load"] = result["serving"]["p99_ms_at_95pct_load"]
         except Exception as e:
             result["serving"] = {"error": repr(e)}
+    if args.calibration_copy:
+        src_buf, dst_buf = cgp_amd.DeviceBuffer(CALIBRATION_BYTES), cgp_amd.DeviceBuffer(CALIBRATION_BYTES)
+        dst_buf.copy_from(src_buf)
+        del src_buf, dst_buf
     print(json.dumps(result), flush=True)
     return 0
 

@@ -227,6 +227,14 @@ class DeviceBuffer:
         check(lib.cgp_copy_to_host(a.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
         return a.reshape(shape) if shape is not None else a
 
+    def copy_from(self, src, nbytes=None):
+        """Device-to-device copy of nbytes (default: all of src) from another
+        DeviceBuffer."""
+        n = src.nbytes if nbytes is None else int(nbytes)
+        if n > self.nbytes or n > src.nbytes:
+            raise ValueError("copy larger than a buffer")
+        check(lib.cgp_copy_device(self.ptr, src.ptr, n))
+
     def view(self, offset, nbytes):
         """A borrowed window [offset, offset+nbytes) of this buffer."""
         if offset < 0 or offset + nbytes > self.nbytes:

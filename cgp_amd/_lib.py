@@ -152,6 +152,7 @@ SIGNATURES = {
     "cgp_device_free": (_st, [_p]),
     "cgp_copy_to_host": (_st, [_p, _p, C.c_uint64]),
     "cgp_copy_to_device": (_st, [_p, _p, C.c_uint64]),
+    "cgp_copy_device": (_st, [_p, _p, C.c_uint64]),
     "cgp_gen_rmat": (_st, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                            C.c_uint32, C.c_uint32, _pp]),
     "cgp_gen_grid": (_st, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, _pp]),

@@ -1897,6 +1897,12 @@ __global__ void k_fill_u32(u32* p, size_t n, u32 val) {
   for (size_t k = n4 * 4 + i; k < n; k += stride) p[k] = val;
 }
 
+// Device-to-device copy of n 16-byte elements (each read and written once).
+__global__ __launch_bounds__(BLOCK) void k_copy16(uint4* dst, const uint4* src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * BLOCK;
+  for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+}
+
 // Fills the first 4 << l4 lanes of each of nrows 64-lane value rows with
 // val (16-byte stores).
 __global__ void k_fill_row_prefix(u32* p, size_t nrows, int l4, u32 val) {
@@ -3004,6 +3010,24 @@ cgp_status cgp_copy_to_host(void* dst, const void* src, uint64_t bytes) {
 
 cgp_status cgp_copy_to_device(void* dst, const void* src, uint64_t bytes) {
   CGP_TRY({ HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+cgp_status cgp_copy_device(void* dst, const void* src, uint64_t bytes) {
+  CGP_TRY({
+    if (bytes == 0) return CGP_OK;
+    if (!dst || !src) cgpi::fail(CGP_EINVAL, "cgp_copy_device: null pointer");
+    if ((((uintptr_t)dst | (uintptr_t)src | bytes) & 15u) != 0) {
+      HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    } else {
+      const size_t n16 = bytes / 16;
+      const int cus = cgpi::device_cus();
+      const size_t blocks = std::min<size_t>((n16 + BLOCK - 1) / BLOCK, (size_t)cus * 16);
+      hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(BLOCK), 0, nullptr, (uint4*)dst, (const uint4*)src,
+                         n16);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipDeviceSynchronize());
+    }
+  });
 }
 
 cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,

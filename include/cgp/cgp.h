@@ -177,6 +177,10 @@ cgp_status cgp_device_alloc(uint64_t bytes, void** out);
 cgp_status cgp_device_free(void* ptr);
 cgp_status cgp_copy_to_host(void* dst_host, const void* src_device, uint64_t bytes);
 cgp_status cgp_copy_to_device(void* dst_device, const void* src_host, uint64_t bytes);
+/* Device-to-device copy (synchronous).  16-byte aligned pointers and sizes run one
+ * library kernel that reads and writes each byte exactly once (a known-traffic
+ * reference for hardware counters); other sizes use the runtime's copy. */
+cgp_status cgp_copy_device(void* dst_device, const void* src_device, uint64_t bytes);
 
 /* ---------------------------------------------------------------- host CSR: generators and loaders */
 /* RMAT (a,b,c, d = 1-a-b-c) with 2^scale vertices and edge_factor * 2^scale edge draws,

@@ -27,25 +27,25 @@ def write_counters(path, counter, rows):
             w.writerow({"Kernel_Name": name, "Counter_Name": counter, "Counter_Value": kib})
 
 
-def test_pmc_traffic_calibrates_on_the_output_kernel(tmp_path):
-    # k_output reads words*V*R*4 bytes and writes out_rows*V*4 bytes.  The
-    # synthetic counters report half of its reads (FETCH factor 2) and all of
-    # its writes (WRITE factor 1), so the advance kernel's calibrated traffic
-    # is 2 x its FETCH plus its WRITE, divided by runs x working launches.
-    V, words, R, out_rows = 1024, 2, 64, 100
-    fetch_out = 4.0 * words * V * R / 2 / 1024  # KiB as the counter reports it
-    write_out = 4.0 * out_rows * V / 1024
+def test_pmc_traffic_calibrates_on_the_copy_kernel(tmp_path):
+    # k_copy16 reads and writes CALIBRATION_BYTES once each.  The synthetic
+    # counters report half of its reads (FETCH factor 2) and all of its writes
+    # (WRITE factor 1), so the advance kernel's calibrated traffic is 2 x its
+    # FETCH plus its WRITE, divided by runs x working launches.
+    cb = bench.CALIBRATION_BYTES
+    fetch_copy = cb / 2 / 1024  # KiB as the counter reports it
+    write_copy = cb / 1024
+    copy = "(anonymous namespace)::k_copy16(HIP_vector_type<unsigned int, 4u>*, HIP_vector_type<unsigned int, 4u> const*, unsigned long)"
     adv = "void (anonymous namespace)::k_adv_dense<2, 256, false, 16, 1>(cgpk::KP)"
     other = "void (anonymous namespace)::k_adv_push<2, 256>(cgpk::KP)"
+    out = "(anonymous namespace)::k_output(unsigned int const*, int)"
     (tmp_path / "fetch").mkdir()
     (tmp_path / "write").mkdir()
     write_counters(tmp_path / "fetch" / "f_counter_collection.csv", "FETCH_SIZE",
-                   [("k_output(unsigned int const*, int)", fetch_out), (adv, 100.0), (adv, 300.0),
-                    (other, 5000.0), ("k_output(unsigned int const*, int)", fetch_out)])
+                   [(copy, fetch_copy), (adv, 100.0), (adv, 300.0), (other, 5000.0), (out, 12345.0)])
     write_counters(tmp_path / "write" / "w_counter_collection.csv", "WRITE_SIZE",
-                   [("k_output(unsigned int const*, int)", write_out), (adv, 40.0), (other, 7000.0)])
-    traffic, factors = bench.pmc_traffic(str(tmp_path), "k_adv_dense<", out_rows, V, words, R,
-                                         runs=2, launches_per_run=1.0)
+                   [(copy, write_copy), (adv, 40.0), (other, 7000.0), (out, 777.0)])
+    traffic, factors = bench.pmc_traffic(str(tmp_path), "k_adv_dense<", runs=2, launches_per_run=1.0)
     assert factors["fetch_size_factor"] == pytest.approx(2.0)
     assert factors["write_size_factor"] == pytest.approx(1.0)
     assert traffic == pytest.approx((2.0 * (100.0 + 300.0) + 40.0) * 1024.0 / 2.0)
@@ -54,7 +54,7 @@ def test_pmc_traffic_calibrates_on_the_output_kernel(tmp_path):
 def test_pmc_traffic_without_calibration_rows_is_none(tmp_path):
     write_counters(tmp_path / "x_counter_collection.csv", "FETCH_SIZE",
                    [("void k_adv_dense<1, 256, false, 16, 1>(KP)", 10.0)])
-    assert bench.pmc_traffic(str(tmp_path), "k_adv_dense<", 1, 64, 1, 64, 1, 1.0) == (None, None)
+    assert bench.pmc_traffic(str(tmp_path), "k_adv_dense<", 1, 1.0) == (None, None)
 
 
 def test_byte_models_from_work_counters():

@@ -133,3 +133,19 @@ def test_per_call_options(gpu):
         en2.set_opts(max_iterations=3)
     assert ei.value.status == cgp_amd.CGP_EBUSY
     en2.serve_stop()
+
+
+def test_device_to_device_copy(gpu):
+    # cgp_copy_device: the library's 16-byte copy kernel for aligned sizes,
+    # the runtime's copy otherwise.
+    a = np.arange(1 << 20, dtype=np.uint32) * np.uint32(2654435761)
+    src = cgp_amd.DeviceBuffer.from_array(a)
+    dst = cgp_amd.DeviceBuffer(a.nbytes)
+    dst.copy_from(src)
+    assert np.array_equal(dst.to_host(), a)
+    odd = cgp_amd.DeviceBuffer(a.nbytes)
+    odd.copy_from(src, a.nbytes - 12)
+    got = odd.to_host()
+    assert np.array_equal(got[: a.size - 3], a[: a.size - 3])
+    with pytest.raises(ValueError):
+        cgp_amd.DeviceBuffer(16).copy_from(src)
