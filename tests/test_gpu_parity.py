@@ -64,6 +64,44 @@ def test_golden_mixed_toy_batches(gpu):
     assert out.tolist() == [[0, 1, 1, 2], [0, 2, 5, 5], [INF, 0, INF, 1], [INF, 0, INF, 3]]
 
 
+@pytest.mark.parametrize("kind", ["min_reduce", "first_write"])
+def test_reduce_op_vectors_through_truncated_runs(gpu, kind):
+    # smoke_reduce_ops.cu:30-144: a value slot under a sequence of candidates,
+    # each entry [slot before, candidate, improved, old value returned].  Here
+    # the slot is vertex t of a small graph whose paths deliver the candidates
+    # in that order, one iteration apart, and runs cut after k iterations
+    # (max_iterations) show the slot after each step.
+    seq = GOLD["reduce_ops"][kind]
+    for before, cand, improved, old in seq:
+        assert old == before and improved == (cand < before if kind == "min_reduce" else before == INF)
+    if kind == "min_reduce":
+        # SSSP from 0 into t = 1: edge 0-1 (10) in iteration 1, 0-2-1 (1 + 4 = 5)
+        # and 0-3-1 (2 + 6 = 8) in iteration 2, 0-4-5-1 (1 + 1 + 1 = 3) in
+        # iteration 3; the slot starts at the golden first value, 10.
+        e = [(0, 1, 10), (0, 2, 1), (2, 1, 4), (0, 3, 2), (3, 1, 6), (0, 4, 1), (4, 5, 1), (5, 1, 1)]
+        expect = {1: seq[0][0], 2: min(seq[0][0], seq[0][1], seq[1][1]), 3: seq[2][1]}
+        kinds, t = [SSSP], 1
+    else:
+        # BFS from 0: t = 5 is first reached at depth 5 along 0-6-7-8-9-5 and
+        # again at depth 7 along 0-1-2-3-4-10-11-5; the first write (the golden
+        # slot's value after its first candidate, 5) stays.  A later smaller
+        # candidate, the golden second step, cannot occur in level order: the
+        # later one here is larger and is rejected in the same way.
+        e = [(0, 6, 1), (6, 7, 1), (7, 8, 1), (8, 9, 1), (9, 5, 1),
+             (0, 1, 1), (1, 2, 1), (2, 3, 1), (3, 4, 1), (4, 10, 1), (10, 11, 1), (11, 5, 1)]
+        expect = {4: INF, 5: seq[0][1], 8: seq[1][0], 0: seq[1][3]}
+        kinds, t = [BFS], 5
+    V = 1 + max(max(a, b) for a, b, _ in e)
+    src = [a for a, b, _ in e] + [b for a, b, _ in e]
+    dst = [b for a, b, _ in e] + [a for a, b, _ in e]
+    w = [x for _, _, x in e] * 2
+    h = cgp_amd.HostCSR.from_coo(V, src, dst, w)
+    g = cgp_amd.Graph.from_host(h)
+    for k, val in expect.items():
+        out = cgp_amd.Engine(g, max_iterations=k).run_batch(kinds, [0])[0]
+        assert int(out[0, t]) == val, (kind, k, int(out[0, t]), val)
+
+
 @pytest.mark.parametrize("mm", GOLD["mm_cases"], ids=lambda c: c["file"])
 def test_golden_matrix_market(gpu, mm):
     h = cgp_amd.HostCSR.load_matrix_market(os.path.join(HERE, "golden", mm["file"]))
