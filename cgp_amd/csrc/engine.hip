@@ -1925,98 +1925,112 @@ struct OutMasks {
   int n_planes;
 };
 
-// Writes the query-major output rows of words 0..Wused-1: out[qmap[w*64+q] *
-// V + v].  SSSP slots take their value rows, transposed through an LDS tile.
-// BFS slots take CGP_INF wherever their visited bit is clear and otherwise
-// their depth: from the depth bit planes when there are any (only the
-// value-row lines holding SSSP slots are then read), else from the value
-// rows.  One workgroup per 64-vertex tile (each output row segment is
-// OUT_VT * 4 = 256 bytes; 128-vertex tiles measured slower: 4.5 vs 2.9 ms on
-// RMAT-22; 16-byte global accesses took 2.9 to 1.55 ms), the words in turn,
-// so the tile's visited records (2W words per vertex) are read from HBM once.
+// Writes the query-major output rows of word w = blockIdx.y: out[qmap[w*64+q]
+// * V + v], through a 64-slot x 64-vertex LDS tile.  SSSP slots take their
+// value rows.  BFS slots take CGP_INF wherever their visited bit is clear and
+// otherwise their depth: from the depth bit planes when there are any (only
+// the value-row lines holding SSSP slots are then read), else from the value
+// rows; each thread resolves sixteen slots of one vertex into the tile, so
+// the store pass only copies the tile out.  One workgroup per (64-vertex
+// tile, word): each output row segment is OUT_VT * 4 = 256 bytes
+// (128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22; 16-byte
+// global accesses took 2.9 to 1.55 ms).
 constexpr int OUT_VT = 64;
 __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* visited, int32_t V, int W,
-                                                  int Wused, int rshift, const int32_t* qmap, OutMasks m,
-                                                  u32* out, int vec) {
+                                                  int rshift, const int32_t* qmap, OutMasks m, u32* out, int vec) {
   __shared__ u32 tile[64][OUT_VT + 1];
   __shared__ u64 s_vis[OUT_VT];
   extern __shared__ u64 s_pl[];  // [n_planes][OUT_VT], sized at launch
   const int R = 1 << rshift;
+  const int w = (int)blockIdx.y;
   const int32_t v0 = (int32_t)blockIdx.x * OUT_VT;
   const int nv = min(OUT_VT, V - v0);
   const int np = m.n_planes;
-  for (int w = 0; w < Wused; ++w) {
-    const u64 bm = m.bfs[w];
-    const u64 rd = np ? m.val[w] : (m.val[w] | bm);  // slots whose value rows are read
-    const size_t rowbase = vrow(V, w, v0, rshift);
-    if (bm) {
-      for (int t = threadIdx.x; t < (np + 1) * OUT_VT; t += BLOCK) {
-        const int b = t / OUT_VT, j = t % OUT_VT;
-        if (j >= nv) continue;
-        if (b == 0) s_vis[j] = visited[(size_t)(v0 + j) * (2 * W) + w];
-        else s_pl[(b - 1) * OUT_VT + j] = m.plane[b - 1][(size_t)w * (size_t)V + (size_t)(v0 + j)];
-      }
+  const u64 bm = m.bfs[w];
+  const u64 rd = np ? m.val[w] : (m.val[w] | bm);  // slots whose value rows are read
+  const size_t rowbase = vrow(V, w, v0, rshift);
+  if (bm) {
+    for (int t = threadIdx.x; t < (np + 1) * OUT_VT; t += BLOCK) {
+      const int b = t / OUT_VT, j = t % OUT_VT;
+      if (j >= nv) continue;
+      if (b == 0) s_vis[j] = visited[(size_t)(v0 + j) * (2 * W) + w];
+      else s_pl[(b - 1) * OUT_VT + j] = m.plane[b - 1][(size_t)w * (size_t)V + (size_t)(v0 + j)];
     }
-    // Value of BFS slot q at tile vertex j (the tile's rows when there are
-    // no planes).
-    auto bfs_val = [&](int q, int j) -> u32 {
-      if (!((s_vis[j] >> q) & 1ull)) return INF;
-      if (!np) return tile[q][j];
-      u32 d = 0;
-      for (int b = 0; b < np; ++b) d |= (u32)((s_pl[b * OUT_VT + j] >> q) & 1ull) << b;
-      return d;
-    };
-    if (vec) {
-      // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by
-      // the launcher): 16-byte loads of 4 slots of a vertex, 16-byte stores
-      // of 4 vertices of a query.
-      const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
-      if ((rd >> c4) & 0xFull) {
-#pragma unroll
-        for (int pass = 0; pass < 4; ++pass) {
-          const int j = pass * 16 + sub;
-          if (j < nv) {
-            const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
-            tile[c4 + 0][j] = x.x;
-            tile[c4 + 1][j] = x.y;
-            tile[c4 + 2][j] = x.z;
-            tile[c4 + 3][j] = x.w;
-          }
-        }
-      }
-      __syncthreads();
+  }
+  if (vec) {
+    // 64-slot rows, V % 4 == 0 and a 16-byte aligned output (checked by the
+    // launcher): 16-byte loads of 4 slots of a vertex.
+    const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
+    if ((rd >> c4) & 0xFull) {
 #pragma unroll
       for (int pass = 0; pass < 4; ++pass) {
-        const int q = pass * 16 + sub;
-        const int32_t dst = qmap[w * 64 + q];
-        if (dst < 0) continue;
-        const bool bfs = (bm >> q) & 1ull;
-        u32 r[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int j = min(c4 + i, OUT_VT - 1);
-          r[i] = bfs ? bfs_val(q, j) : tile[q][j];
+        const int j = pass * 16 + sub;
+        if (j < nv) {
+          const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
+          tile[c4 + 0][j] = x.x;
+          tile[c4 + 1][j] = x.y;
+          tile[c4 + 2][j] = x.z;
+          tile[c4 + 3][j] = x.w;
         }
-        u32* o = out + (size_t)dst * V + v0 + c4;
-        if (c4 + 4 <= nv) {
-          *(uint4*)o = make_uint4(r[0], r[1], r[2], r[3]);
-        } else {
-          for (int i = 0; c4 + i < nv; ++i) o[i] = r[i];
-        }
-      }
-    } else {
-      for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
-        const int j = t >> rshift, q = t & (R - 1);
-        if (j < nv && ((rd >> q) & 1ull)) tile[q][j] = vals[rowbase + (size_t)t];
-      }
-      __syncthreads();
-      for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
-        const int q = t / OUT_VT, j = t % OUT_VT;
-        const int32_t dst = qmap[w * 64 + q];
-        if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = ((bm >> q) & 1ull) ? bfs_val(q, j) : tile[q][j];
       }
     }
-    __syncthreads();  // the tile, s_vis and s_pl are rewritten for the next word
+  } else {
+    for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
+      const int j = t >> rshift, q = t & (R - 1);
+      if (j < nv && ((rd >> q) & 1ull)) tile[q][j] = vals[rowbase + (size_t)t];
+    }
+  }
+  __syncthreads();
+  if (bm) {
+    // Thread (j, g): BFS slots 16g..16g+15 of vertex j into the tile.
+    const int j = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
+    if (j < nv && g * 16 < R) {
+      const u64 vis = s_vis[j];
+      // The vertex's first eight planes in registers (depths below 256);
+      // deeper planes are read from LDS.
+      constexpr int NPR = 8;
+      u64 pl[NPR];
+#pragma unroll
+      for (int b = 0; b < NPR; ++b) pl[b] = b < np ? s_pl[b * OUT_VT + j] : 0ull;
+      for (int q = g * 16; q < min(R, g * 16 + 16); ++q) {
+        if (!((bm >> q) & 1ull)) continue;
+        u32 d;
+        if (!((vis >> q) & 1ull)) {
+          d = INF;
+        } else if (np) {
+          d = 0;
+#pragma unroll
+          for (int b = 0; b < NPR; ++b) d |= (u32)((pl[b] >> q) & 1ull) << b;
+          for (int b = NPR; b < np; ++b) d |= (u32)((s_pl[b * OUT_VT + j] >> q) & 1ull) << b;
+        } else {
+          d = tile[q][j];
+        }
+        tile[q][j] = d;
+      }
+    }
+    __syncthreads();
+  }
+  if (vec) {
+    // 16-byte stores of 4 vertices of a query.
+    const int sub = (int)(threadIdx.x >> 4), c4 = (int)(threadIdx.x & 15) * 4;
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int q = pass * 16 + sub;
+      const int32_t dst = qmap[w * 64 + q];
+      if (dst < 0) continue;
+      u32* o = out + (size_t)dst * V + v0 + c4;
+      if (c4 + 4 <= nv) {
+        *(uint4*)o = make_uint4(tile[q][c4], tile[q][c4 + 1], tile[q][c4 + 2], tile[q][c4 + 3]);
+      } else {
+        for (int i = 0; c4 + i < nv; ++i) o[i] = tile[q][c4 + i];
+      }
+    }
+  } else {
+    for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
+      const int q = t / OUT_VT, j = t % OUT_VT;
+      const int32_t dst = qmap[w * 64 + q];
+      if (dst >= 0 && j < nv) out[(size_t)dst * V + v0 + j] = tile[q][j];
+    }
   }
 }
 
@@ -2773,9 +2787,9 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     }
     om.n_planes = cfg.depth_planes ? en->plane_count : 0;
     for (int b = 0; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
-    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT), dim3(BLOCK),
+    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK),
                        (size_t)om.n_planes * OUT_VT * sizeof(u64), en->s, (const u32*)en->vals.p,
-                       (const u64*)en->visited.p, V, W, Wused, rshift, (const int32_t*)en->qmap.p, om, dout, vec);
+                       (const u64*)en->visited.p, V, W, rshift, (const int32_t*)en->qmap.p, om, dout, vec);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
   }
