@@ -3289,10 +3289,11 @@ cgp_engine_opts resolve_opts(const cgp_graph* g, const cgp_engine_opts* opts, co
   if (o.dense_threshold == 0) {
     // Automatic lane-per-query threshold.  The table loop (postponing off)
     // costs less per edge row than the postponing loop, so it pays from
-    // fewer active queries: 6 took the RMAT-22 mixed batch from 68.3 to
-    // 66.5 ms, while on the 4096^2 grid (postponing on) 6 instead of 8
-    // took 2380 to 2535 ms.
-    o.dense_threshold = o.sssp_delta == 0 ? 6 : 8;
+    // fewer active queries: on the RMAT-22 mixed batch 5 measured 53.4
+    // against 54.2 ms for 6 (three interleaved runs each; 4: 55.2, 8: 55.2),
+    // while on the 4096^2 grid (postponing on) 6 instead of 8 took 2380 to
+    // 2535 ms.
+    o.dense_threshold = o.sssp_delta == 0 ? 5 : 8;
   }
   return o;
 }

@@ -292,7 +292,7 @@ def test_near_far_automatic_off_on_small_world(gpu):
     h, g, og = rmat(12, 16, 4, 1, 255)
     en = cgp_amd.Engine(g)
     assert en.opts["sssp_delta"] == 0
-    assert en.opts["dense_threshold"] == 6  # automatic threshold with postponing off
+    assert en.opts["dense_threshold"] == 5  # automatic threshold with postponing off
     assert cgp_amd.Engine(g, dense_threshold=3).opts["dense_threshold"] == 3
 
 
