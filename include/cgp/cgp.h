@@ -102,7 +102,7 @@ typedef struct {
                                   a symmetric graph (SSSP: with symmetric weights) and, for SSSP,
                                   postponing off (sssp_delta 0) */
   int32_t advance_blocks;      /* workgroups of the advance launches (256-thread equivalents);
-                                  0 = one per 8192 edges, between 1 and 8 per CU */
+                                  0 = one per 65536 edges, between 1 and 8 per CU */
   int32_t sssp_delta;          /* SSSP near-far step: frontier pairs whose value is >= a threshold
                                   wait; the threshold moves to the smallest waiting value + delta
                                   when nothing below it is left.  0 = no waiting (Bellman-Ford
