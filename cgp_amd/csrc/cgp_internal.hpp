@@ -78,6 +78,10 @@ bool is_symmetric(const HostCSR& g);
 // Every edge u -> v of weight w has a reverse edge v -> u of weight w
 // (structure symmetric and weighted; checked per edge).
 bool weights_symmetric(const HostCSR& g);
+// The graph with its vertices renumbered by descending out-degree (ties in id
+// order): perm[new] = old, inv[old] = new; each row's targets renumbered and
+// sorted (weights follow their edges).
+HostCSR relabel_by_degree(const HostCSR& g, std::vector<int32_t>& perm, std::vector<int32_t>& inv);
 
 }  // namespace cgpi
 

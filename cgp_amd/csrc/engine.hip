@@ -1636,12 +1636,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
       // into plane b (one 8-byte atomic per word with new bits, no return).
       if (cfg->depth_planes) {
         for (u32 bits = depth_base; bits; bits &= bits - 1) {
-          u64* const pl = cfg->plane[__builtin_ctz(bits)];
+          const int b = __builtin_ctz(bits);
 #pragma unroll
           for (int w = 0; w < W; ++w)
-            if (nb[w])
-              __hip_atomic_fetch_or(&pl[(size_t)w * (size_t)P.V + (size_t)v], nb[w], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+            if (nb[w]) {
+              const size_t wv = (size_t)w * (size_t)P.V + (size_t)v;
+              u64* const a = b < 4 ? cfg->plane4 + (wv << 2) + b : cfg->plane[b] + wv;
+              __hip_atomic_fetch_or(a, nb[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
       } else if (rshift == 6) {
         // Value rows (serving, allocation-free runs): lanes of the wave write
@@ -1921,25 +1923,29 @@ __global__ void k_fill_row_prefix(u32* p, size_t nrows, int l4, u32 val) {
 struct OutMasks {
   u64 bfs[MAXW];
   u64 val[MAXW];
-  const u64* plane[MAXPLANES];  // BFS depth bit planes (n_planes of them; 0: depths in the value rows)
-  int n_planes;
+  const u64* plane4;            // depth planes 0-3, in 32-byte (word, vertex) blocks
+  const u64* plane[MAXPLANES];  // depth planes 4.. (plane-major)
+  int n_planes;                 // planes in use (0: depths in the value rows)
 };
 
 // Writes the query-major output rows of word w = blockIdx.y: out[qmap[w*64+q]
-// * V + v], through a 64-slot x 64-vertex LDS tile.  SSSP slots take their
-// value rows.  BFS slots take CGP_INF wherever their visited bit is clear and
-// otherwise their depth: from the depth bit planes when there are any (only
-// the value-row lines holding SSSP slots are then read), else from the value
-// rows; each thread resolves sixteen slots of one vertex into the tile, so
-// the store pass only copies the tile out.  One workgroup per (64-vertex
-// tile, word): each output row segment is OUT_VT * 4 = 256 bytes
-// (128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22; 16-byte
-// global accesses took 2.9 to 1.55 ms).
+// * V + v], through a 64-slot x 64-vertex LDS tile, v in the caller's
+// numbering (the tile's internal vertices are inv[v]; inv null: the same).
+// SSSP slots take their value rows.  BFS slots take CGP_INF wherever their
+// visited bit is clear and otherwise their depth: from the depth bit planes
+// when there are any (only the value-row lines holding SSSP slots are then
+// read), else from the value rows; each thread resolves sixteen slots of one
+// vertex into the tile, so the store pass only copies the tile out.  One
+// workgroup per (64-vertex tile, word): each output row segment is OUT_VT *
+// 4 = 256 bytes (128-vertex tiles measured slower: 4.5 vs 2.9 ms on RMAT-22;
+// 16-byte global accesses took 2.9 to 1.55 ms).
 constexpr int OUT_VT = 64;
-__global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* visited, int32_t V, int W,
-                                                  int rshift, const int32_t* qmap, OutMasks m, u32* out, int vec) {
+__global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* visited, const int32_t* inv,
+                                                  int32_t V, int W, int rshift, const int32_t* qmap, OutMasks m,
+                                                  u32* out, int vec) {
   __shared__ u32 tile[64][OUT_VT + 1];
   __shared__ u64 s_vis[OUT_VT];
+  __shared__ int32_t s_u[OUT_VT];
   extern __shared__ u64 s_pl[];  // [n_planes][OUT_VT], sized at launch
   const int R = 1 << rshift;
   const int w = (int)blockIdx.y;
@@ -1948,13 +1954,19 @@ __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* vi
   const int np = m.n_planes;
   const u64 bm = m.bfs[w];
   const u64 rd = np ? m.val[w] : (m.val[w] | bm);  // slots whose value rows are read
-  const size_t rowbase = vrow(V, w, v0, rshift);
+  if ((int)threadIdx.x < OUT_VT) {
+    const int j = (int)threadIdx.x;
+    s_u[j] = j < nv ? (inv ? inv[v0 + j] : v0 + j) : 0;
+  }
+  __syncthreads();
   if (bm) {
     for (int t = threadIdx.x; t < (np + 1) * OUT_VT; t += BLOCK) {
       const int b = t / OUT_VT, j = t % OUT_VT;
       if (j >= nv) continue;
-      if (b == 0) s_vis[j] = visited[(size_t)(v0 + j) * (2 * W) + w];
-      else s_pl[(b - 1) * OUT_VT + j] = m.plane[b - 1][(size_t)w * (size_t)V + (size_t)(v0 + j)];
+      const size_t u = (size_t)s_u[j];
+      if (b == 0) s_vis[j] = visited[u * (2 * W) + w];
+      else if (b <= 4) s_pl[(b - 1) * OUT_VT + j] = m.plane4[(((size_t)w * (size_t)V + u) << 2) + (b - 1)];
+      else s_pl[(b - 1) * OUT_VT + j] = m.plane[b - 1][(size_t)w * (size_t)V + u];
     }
   }
   if (vec) {
@@ -1966,7 +1978,7 @@ __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* vi
       for (int pass = 0; pass < 4; ++pass) {
         const int j = pass * 16 + sub;
         if (j < nv) {
-          const uint4 x = *(const uint4*)(vals + rowbase + (size_t)j * 64 + c4);
+          const uint4 x = *(const uint4*)(vals + vrow(V, w, s_u[j], 6) + c4);
           tile[c4 + 0][j] = x.x;
           tile[c4 + 1][j] = x.y;
           tile[c4 + 2][j] = x.z;
@@ -1977,7 +1989,7 @@ __global__ __launch_bounds__(BLOCK) void k_output(const u32* vals, const u64* vi
   } else {
     for (int t = threadIdx.x; t < OUT_VT * R; t += BLOCK) {
       const int j = t >> rshift, q = t & (R - 1);
-      if (j < nv && ((rd >> q) & 1ull)) tile[q][j] = vals[rowbase + (size_t)t];
+      if (j < nv && ((rd >> q) & 1ull)) tile[q][j] = vals[vrow(V, w, s_u[j], rshift) + (size_t)q];
     }
   }
   __syncthreads();
@@ -2217,20 +2229,57 @@ __global__ void k_wcc_compress(int32_t* parent, int32_t V) {
   }
 }
 
-// Writes the component labels into every WCC output row.
-__global__ void k_wcc_rows(const int32_t* parent, int32_t V, const int32_t* rows_out, int nrows, u32* out) {
+// Renumbered graphs: minid[root] = the smallest caller id in the root's
+// component (minid starts at INT32_MAX).  Caller ids are taken in order; the
+// lanes of a wave that share the root of its first lane in a component of
+// more than one vertex (the large component, on the graphs that have one)
+// reduce their minimum first, and every update is skipped when the current
+// minimum is already smaller, so the large component's root is not hit by
+// one atomic per vertex.
+__global__ __launch_bounds__(BLOCK) void k_wcc_minid(const int32_t* parent, const int32_t* inv, int32_t V,
+                                                     int32_t* minid) {
+  for (int64_t b = (int64_t)blockIdx.x * BLOCK; b < V; b += (int64_t)gridDim.x * BLOCK) {
+    const int64_t v = b + threadIdx.x;
+    const bool ok = v < V;
+    const int32_t u = ok ? inv[v] : 0;
+    const int32_t r = ok ? parent[u] : -1;
+    const u64 shared = __ballot(ok && r != u);
+    const int32_t rd = shared ? __builtin_amdgcn_readlane(r, __builtin_ctzll(shared)) : -1;
+    const bool mine = ok && r == rd;
+    int32_t m = mine ? (int32_t)v : 0x7FFFFFFF;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = min(m, __shfl_xor(m, d, 64));
+    if (mine && (int64_t)__builtin_amdgcn_readfirstlane(__builtin_ctzll(__ballot(mine))) == (int64_t)lane_id()) {
+      if (__hip_atomic_load(&minid[rd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > m) atomicMin(&minid[rd], m);
+    }
+    if (ok && !mine && __hip_atomic_load(&minid[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > (int32_t)v)
+      atomicMin(&minid[r], (int32_t)v);
+  }
+}
+
+// Writes the component labels (the smallest vertex id of each component, in
+// the caller's numbering) into every WCC output row (rows_out null: one row
+// at out).  Vertex v of the caller is inv[v] inside (inv null: the same);
+// minid (null: the root itself) maps an internal root to its label.
+__global__ void k_wcc_rows(const int32_t* parent, const int32_t* minid, const int32_t* inv, int32_t V,
+                           const int32_t* rows_out, int nrows, u32* out) {
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
-    const u32 l = (u32)parent[v];
-    for (int r = 0; r < nrows; ++r) out[(size_t)rows_out[r] * V + v] = l;
+    const int32_t root = parent[inv ? inv[v] : (int32_t)v];
+    const u32 l = (u32)(minid ? minid[root] : root);
+    for (int r = 0; r < nrows; ++r) out[(size_t)(rows_out ? rows_out[r] : 0) * V + v] = l;
   }
 }
 
 // m_q per output row: sum of out-degrees over vertices with a finite value.
-__global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, int32_t V, const u32* out, u64* m) {
+__global__ __launch_bounds__(BLOCK) void k_reached(const int32_t* off, const int32_t* inv, int32_t V, const u32* out,
+                                                   u64* m) {
   const int r = blockIdx.y;
   u64 acc = 0;
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK)
-    if (out[(size_t)r * V + v] != INF) acc += (u64)(off[v + 1] - off[v]);
+    if (out[(size_t)r * V + v] != INF) {
+      const int32_t u = inv ? inv[v] : (int32_t)v;  // the caller's vertex v inside the graph
+      acc += (u64)(off[u + 1] - off[u]);
+    }
   acc = wave_sum64(acc);
   if (lane_id() == 0 && acc) atomicAdd(&m[r], acc);
 }
@@ -2466,9 +2515,22 @@ void compute_wcc(cgp_engine* en) {
   HIP_CHECK(hipGetLastError());
 }
 
+// Writes the labels of compute_wcc's forest into nrows output rows (d_rows
+// null: one row at out).  On renumbered graphs the smallest caller id of each
+// component is found first, in the touched list's buffer (free between
+// iterations).
 void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) {
+  const cgp_graph* g = en->g;
+  int32_t* minid = nullptr;
+  if (g->inv) {
+    en->touched.ensure((size_t)g->V * 4 + 4);
+    minid = (int32_t*)en->touched.p;
+    fill_u32(en->s, minid, (size_t)g->V, 0x7FFFFFFFu, en->cus);
+    hipLaunchKernelGGL(k_wcc_minid, dim3(en->cus * 16), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p,
+                       (const int32_t*)g->inv, g->V, minid);
+  }
   hipLaunchKernelGGL(k_wcc_rows, dim3(en->cus * 16), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p,
-                     en->g->V, d_rows, nrows, out);
+                     (const int32_t*)minid, (const int32_t*)g->inv, g->V, d_rows, nrows, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -2484,6 +2546,21 @@ static void add_depth_planes(cgp_engine* en, int W, int need, bool live) {
   Cfg& cfg = *en->h_cfg;
   need = std::min(need, MAXPLANES);
   for (int b = en->plane_count; b < need; ++b) {
+    if (b < 4) {
+      // Planes 0-3 share 32-byte (word, vertex) blocks: one allocation,
+      // zeroed over the BFS words with the first of them.
+      if (b == 0) {
+        en->plane4.ensure(V * (size_t)W * 32);
+        u64* p4 = (u64*)en->plane4.p;
+        for (int w = 0; w < W; ++w)
+          if ((en->plane_words >> w) & 1u) fill_u32(en->s, p4 + (size_t)w * V * 4, V * 8, 0u, en->cus);
+        cfg.plane4 = p4;
+        if (live)
+          HIP_CHECK(hipMemcpyAsync((char*)en->d_cfg + offsetof(Cfg, plane4), &cfg.plane4, sizeof(u64*),
+                                   hipMemcpyHostToDevice, en->s));
+      }
+      continue;
+    }
     en->plane[b].ensure(V * (size_t)W * 8);
     u64* pl = (u64*)en->plane[b].p;
     for (int w = 0; w < W; ++w)
@@ -2655,7 +2732,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     for (int k = 0; k < ns; ++k) {
       const int32_t qi = order[p0 + k];
       const int w = k >> 6, q = k & 63;
-      h_src[k] = queries[qi].source;
+      h_src[k] = g->internal(queries[qi].source);
       h_qmap[k] = qi;
       if (!no_alloc) en->rep_slot[(size_t)qi] = (int32_t)((p0 / slots_max) << 16 | w << 8 | q);
       if (queries[qi].kind == CGP_BFS) {
@@ -2786,10 +2863,12 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       om.val[w] = cfg.val_bits[w];
     }
     om.n_planes = cfg.depth_planes ? en->plane_count : 0;
-    for (int b = 0; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
+    om.plane4 = (const u64*)en->plane4.p;
+    for (int b = 4; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
     hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK),
                        (size_t)om.n_planes * OUT_VT * sizeof(u64), en->s, (const u32*)en->vals.p,
-                       (const u64*)en->visited.p, V, W, rshift, (const int32_t*)en->qmap.p, om, dout, vec);
+                       (const u64*)en->visited.p, (const int32_t*)g->inv, V, W, rshift,
+                       (const int32_t*)en->qmap.p, om, dout, vec);
     HIP_CHECK(hipGetLastError());
     en->tick(CAT_OUT, false);
   }
@@ -3074,18 +3153,31 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
       for (int64_t e = 0; e < E; ++e) sum += wt[e];
       g->mean_weight = sum / (double)E;
     }
+    // Stored with the vertices renumbered by descending degree: the hubs
+    // that most edges reach get neighbouring value rows and records, which
+    // share DRAM pages, TLB entries and cache sets (RMAT-22, 160 BFS + 80
+    // SSSP + 16 WCC: 53.6 -> 48.3 ms per batch).  Results, sources and WCC
+    // labels stay in the caller's numbering.
+    std::vector<int32_t> perm;
+    h = cgpi::relabel_by_degree(h, perm, g->h_inv);
     HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
     HIP_CHECK(hipMalloc((void**)&g->idx, (size_t)E * 4 + 4));
     HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
+    HIP_CHECK(hipMalloc((void**)&g->perm, (size_t)V * 4 + 4));
+    HIP_CHECK(hipMalloc((void**)&g->inv, (size_t)V * 4 + 4));
     if (wt) HIP_CHECK(hipMalloc((void**)&g->wt, (size_t)E * 4 + 4));
-    HIP_CHECK(hipMemcpy(g->off, off, ((size_t)V + 1) * 4, hipMemcpyHostToDevice));
-    if (E) HIP_CHECK(hipMemcpy(g->idx, idx, (size_t)E * 4, hipMemcpyHostToDevice));
-    if (wt && E) HIP_CHECK(hipMemcpy(g->wt, wt, (size_t)E * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(g->off, h.off.data(), ((size_t)V + 1) * 4, hipMemcpyHostToDevice));
+    if (E) HIP_CHECK(hipMemcpy(g->idx, h.idx.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+    if (wt && E) HIP_CHECK(hipMemcpy(g->wt, h.wt.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+    if (V) {
+      HIP_CHECK(hipMemcpy(g->perm, perm.data(), (size_t)V * 4, hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(g->inv, g->h_inv.data(), (size_t)V * 4, hipMemcpyHostToDevice));
+    }
     {
       std::vector<int32_t> lo;
       std::vector<int32_t> hi;
       for (int32_t v = 0; v < V; ++v) {
-        const int32_t d = off[v + 1] - off[v];
+        const int32_t d = h.off[v + 1] - h.off[v];
         if (d > PULL_WAVE_DEG) hi.push_back(v);
         else if (d > 0) lo.push_back(v);
       }
@@ -3205,6 +3297,8 @@ void cgp_graph_destroy(cgp_graph* g) {
   (void)hipFree(g->rows);
   (void)hipFree(g->pull_lo);
   (void)hipFree(g->pull_hi);
+  if (g->perm) (void)hipFree(g->perm);
+  if (g->inv) (void)hipFree(g->inv);
   delete g;
 }
 
@@ -3351,6 +3445,7 @@ void cgp_engine_destroy(cgp_engine* en) {
                     &en->outbuf, &en->qmap, &en->slot_src, &en->iters, &en->pull_items};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : en->plane) b.release();
+  en->plane4.release();
   for (hipEvent_t e : en->ev_pool) (void)hipEventDestroy(e);
   if (en->d_ctl) (void)hipFree(en->d_ctl);
   if (en->d_cfg) (void)hipFree(en->d_cfg);
@@ -3716,7 +3811,8 @@ cgp_status cgp_reached_edges(const cgp_graph* g, const uint32_t* out_dev, int32_
     HIP_CHECK(hipMalloc((void**)&dm, sizeof(u64) * (size_t)n));
     HIP_CHECK(hipMemset(dm, 0, sizeof(u64) * (size_t)n));
     const int cus = device_cus();
-    hipLaunchKernelGGL(k_reached, dim3(cus * 2, n), dim3(BLOCK), 0, 0, (const int32_t*)g->off, g->V, out_dev, dm);
+    hipLaunchKernelGGL(k_reached, dim3(cus * 2, n), dim3(BLOCK), 0, 0, (const int32_t*)g->off, (const int32_t*)g->inv,
+                       g->V, out_dev, dm);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(m_out, dm, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost);
     (void)hipFree(dm);

@@ -49,6 +49,14 @@ struct cgp_graph {
   int32_t* pull_hi = nullptr;
   int32_t n_pull_lo = 0;
   int32_t n_pull_hi = 0;
+  // Graphs built from host arrays are stored with their vertices renumbered
+  // by descending degree: perm[internal] = caller id, inv[caller id] =
+  // internal (device arrays; null for graphs over caller device arrays, which
+  // keep the caller's numbering), and inv on the host for query sources.
+  int32_t* perm = nullptr;
+  int32_t* inv = nullptr;
+  std::vector<int32_t> h_inv;
+  int32_t internal(int32_t v) const { return h_inv.empty() ? v : h_inv[(size_t)v]; }
 };
 
 namespace cgpi {
@@ -129,7 +137,8 @@ struct cgp_engine {
   cgpi::DevBuf parent, wcc_rows, wcc_probe, outbuf, qmap, slot_src;
   // BFS depth bit planes of the running pass (batch runs): plane_count
   // planes zeroed over the pass's BFS words (plane_words bit w).
-  cgpi::DevBuf plane[cgpk::MAXPLANES];
+  cgpi::DevBuf plane4;                  // planes 0-3 in (word, vertex) blocks
+  cgpi::DevBuf plane[cgpk::MAXPLANES];  // planes 4..
   int plane_count = 0;
   u32 plane_words = 0;
   cgpi::DevBuf pull_items;  // SSSP pull items of the graph, built on first use

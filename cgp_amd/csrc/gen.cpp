@@ -430,4 +430,45 @@ bool weights_symmetric(const HostCSR& g) {
   return ok.load();
 }
 
+HostCSR relabel_by_degree(const HostCSR& g, std::vector<int32_t>& perm, std::vector<int32_t>& inv) {
+  const int32_t V = g.V;
+  perm.assign((size_t)V, 0);
+  inv.assign((size_t)V, 0);
+  // Counting sort by descending degree, stable in id order.
+  int32_t maxd = 0;
+  for (int32_t v = 0; v < V; ++v) maxd = std::max(maxd, g.off[v + 1] - g.off[v]);
+  std::vector<int64_t> start((size_t)maxd + 2, 0);
+  for (int32_t v = 0; v < V; ++v) ++start[(size_t)(maxd - (g.off[v + 1] - g.off[v])) + 1];
+  for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+  for (int32_t v = 0; v < V; ++v) {
+    const int64_t i = start[(size_t)(maxd - (g.off[v + 1] - g.off[v]))]++;
+    perm[(size_t)i] = v;
+    inv[(size_t)v] = (int32_t)i;
+  }
+  HostCSR r;
+  r.V = V;
+  r.E = g.E;
+  r.off.assign((size_t)V + 1, 0);
+  for (int32_t i = 0; i < V; ++i) r.off[(size_t)i + 1] = r.off[(size_t)i] + (g.off[perm[i] + 1] - g.off[perm[i]]);
+  r.idx.resize((size_t)g.E);
+  const bool w = !g.wt.empty();
+  if (w) r.wt.resize((size_t)g.E);
+  parallel_for(V, [&](int64_t b, int64_t e, int) {
+    std::vector<std::pair<int32_t, uint32_t>> row;
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t o = perm[(size_t)i];
+      const int32_t s0 = g.off[o], s1 = g.off[o + 1];
+      row.resize((size_t)(s1 - s0));
+      for (int32_t k = s0; k < s1; ++k) row[(size_t)(k - s0)] = {inv[(size_t)g.idx[k]], w ? g.wt[k] : 0u};
+      std::sort(row.begin(), row.end());
+      const int32_t d0 = r.off[(size_t)i];
+      for (size_t k = 0; k < row.size(); ++k) {
+        r.idx[(size_t)d0 + k] = row[k].first;
+        if (w) r.wt[(size_t)d0 + k] = row[k].second;
+      }
+    }
+  });
+  return r;
+}
+
 }  // namespace cgpi

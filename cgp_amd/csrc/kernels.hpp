@@ -85,12 +85,15 @@ struct Cfg {
   u64 spush_pairs_thr;  // back to push when the pairs fall below this
   // Batch runs (every slot starts at iteration 0, so a compaction's new BFS
   // bits all get the same depth): depths are kept as bit planes, bit q of
-  // plane[b][w * V + v] = bit b of the depth of slot 64w+q at vertex v,
+  // plane b's word for (w, v) = bit b of the depth of slot 64w+q at vertex v,
   // written with one OR per set bit of the depth and per word with new bits.
+  // The first four planes of a (word, vertex) share one 32-byte block, so
+  // reading a vertex's depths (out of vertex order) touches one sector.
   // Planes below plane_count exist and start at zero; more are added by the
   // host before the depths need them.  0: depths go into the value rows.
   int32_t depth_planes;
-  u64* plane[MAXPLANES];
+  u64* plane4;             // planes 0-3: plane4[((w * V + v) << 2) + b]
+  u64* plane[MAXPLANES];   // planes 4..: plane[b][w * V + v]
 };
 
 // A device counter on a 256-byte line of its own: the work cursors and

@@ -39,17 +39,19 @@ struct SnapArgs {
 // column into out[row * V + v] (when out is set), then resets the column to
 // INF and clears the slot's visited bit, leaving the slot ready for its next
 // query.
-__global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visited, int32_t V, int W,
-                                                          int rshift, SnapArgs a, u32* out) {
+__global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visited, const int32_t* inv, int32_t V,
+                                                          int W, int rshift, SnapArgs a, u32* out) {
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    // The caller's vertex v is u inside the graph (inv null: the same).
+    const size_t u = (size_t)(inv ? inv[v] : (int32_t)v);
     for (int k = 0; k < a.n; ++k) {
       const u32 e = a.ent[k];
-      const size_t at = (((size_t)(e >> 24) * (size_t)V + (size_t)v) << rshift) + ((e >> 16) & 0xFFu);
+      const size_t at = (((size_t)(e >> 24) * (size_t)V + u) << rshift) + ((e >> 16) & 0xFFu);
       if (out) out[(size_t)(e & 0xFFFFu) * V + v] = vals[at];
       vals[at] = INF;
     }
     for (int w = 0; w < W; ++w)
-      if (a.clear[w]) visited[(size_t)v * (2 * W) + w] &= ~a.clear[w];  // visited half of the record
+      if (a.clear[w]) visited[u * (2 * W) + w] &= ~a.clear[w];  // visited half of the record
   }
 }
 
@@ -86,6 +88,7 @@ struct ServeState {
   bool wcc_ready = false;
   std::vector<u32> wcc_host;
   cgpi::DevBuf d_src;  // seed sources
+  cgpi::DevBuf wcc_lab;  // WCC labels of the session's graph (caller numbering)
   // The pinned staging that admissions upload from (slot configuration and
   // seed sources) is rewritten only after the previous upload has run.
   hipEvent_t staged = nullptr;
@@ -153,8 +156,10 @@ void admit(cgp_engine* en, ServeState* ss) {
       ss->queue.pop_front();
       if (!ss->wcc_ready) {
         compute_wcc(en);
+        ss->wcc_lab.ensure((size_t)V * 4 + 4);
+        write_wcc_rows(en, nullptr, 1, (u32*)ss->wcc_lab.p);
         ss->wcc_host.resize((size_t)V);
-        HIP_CHECK(hipMemcpyAsync(ss->wcc_host.data(), en->parent.p, (size_t)V * 4, hipMemcpyDeviceToHost, en->s));
+        HIP_CHECK(hipMemcpyAsync(ss->wcc_host.data(), ss->wcc_lab.p, (size_t)V * 4, hipMemcpyDeviceToHost, en->s));
         HIP_CHECK(hipStreamSynchronize(en->s));
         ss->wcc_ready = true;
       }
@@ -208,7 +213,7 @@ void admit(cgp_engine* en, ServeState* ss) {
     if (pd.q.kind == CGP_BFS) cfg.bfs_bits[w] |= bit;
     else cfg.val_bits[w] |= bit;
     cfg.start[slot] = (int32_t)st.start;
-    h_src[slot] = pd.q.source;
+    h_src[slot] = en->g->internal(pd.q.source);
     ++ss->busy;
     any = true;
   }
@@ -280,7 +285,8 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
       a.clear[slot >> 6] |= 1ull << (slot & 63);
     }
     hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
-                       (u64*)en->visited.p, V, ss->W, ss->rshift, a, staging ? (u32*)staging->p : nullptr);
+                       (u64*)en->visited.p, (const int32_t*)en->g->inv, V, ss->W, ss->rshift, a,
+                       staging ? (u32*)staging->p : nullptr);
     HIP_CHECK(hipGetLastError());
   }
   std::vector<cgp_completion> comps((size_t)k);
@@ -338,6 +344,7 @@ void serve_free(cgp_engine* en) {
     delete b;
   }
   ss->d_src.release();
+  ss->wcc_lab.release();
   // Stamps used by the session must not recur in later runs; slots may hold
   // bits in next[] when stopped mid-run.
   en->stamp_epoch += ss->iter + 2;
