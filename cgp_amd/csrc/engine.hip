@@ -1547,7 +1547,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter(KP P) {
   const Cfg* cfg = P.cfg;
   const u32 iter = C->iter, p = iter & 1u, np = p ^ 1u;
   const bool scan = C->dense_mode != 0;
-  const u32 n = scan ? (u32)P.V : C->tcount.v;
+  const u32 n = scan ? (u32)P.scan_v : C->tcount.v;
   const int rshift = cfg->rshift, dense_t = cfg->dense_t;
   const int lane = lane_id();
   u64 bb[W], vb[W], un[W];
@@ -3153,26 +3153,38 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
       for (int64_t e = 0; e < E; ++e) sum += wt[e];
       g->mean_weight = sum / (double)E;
     }
-    // Stored with the vertices renumbered by descending degree: the hubs
-    // that most edges reach get neighbouring value rows and records, which
-    // share DRAM pages, TLB entries and cache sets (RMAT-22, 160 BFS + 80
-    // SSSP + 16 WCC: 53.6 -> 48.3 ms per batch).  Results, sources and WCC
-    // labels stay in the caller's numbering.
+    // Skewed graphs are stored with their vertices renumbered by descending
+    // degree: the hubs that most edges reach get neighbouring value rows and
+    // records, which share DRAM pages, TLB entries and cache sets (RMAT-22,
+    // 160 BFS + 80 SSSP + 16 WCC: 54.8 -> 50.2 ms per batch).  Graphs whose
+    // largest degree is within 32x the mean (meshes, grids) keep their
+    // numbering, whose neighbours are already close.  Results, sources and
+    // WCC labels stay in the caller's numbering.
+    int32_t maxd = 0;
+    for (int32_t v = 0; v < V; ++v) maxd = std::max(maxd, off[v + 1] - off[v]);
     std::vector<int32_t> perm;
-    h = cgpi::relabel_by_degree(h, perm, g->h_inv);
+    if (V > 0 && (double)maxd >= 32.0 * std::max(1.0, (double)E / (double)V))
+      h = cgpi::relabel_by_degree(h, perm, g->h_inv);
     HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
     HIP_CHECK(hipMalloc((void**)&g->idx, (size_t)E * 4 + 4));
     HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
-    HIP_CHECK(hipMalloc((void**)&g->perm, (size_t)V * 4 + 4));
-    HIP_CHECK(hipMalloc((void**)&g->inv, (size_t)V * 4 + 4));
+    if (!perm.empty()) {
+      HIP_CHECK(hipMalloc((void**)&g->perm, (size_t)V * 4 + 4));
+      HIP_CHECK(hipMalloc((void**)&g->inv, (size_t)V * 4 + 4));
+    }
     if (wt) HIP_CHECK(hipMalloc((void**)&g->wt, (size_t)E * 4 + 4));
     HIP_CHECK(hipMemcpy(g->off, h.off.data(), ((size_t)V + 1) * 4, hipMemcpyHostToDevice));
     if (E) HIP_CHECK(hipMemcpy(g->idx, h.idx.data(), (size_t)E * 4, hipMemcpyHostToDevice));
     if (wt && E) HIP_CHECK(hipMemcpy(g->wt, h.wt.data(), (size_t)E * 4, hipMemcpyHostToDevice));
-    if (V) {
+    if (!perm.empty()) {
       HIP_CHECK(hipMemcpy(g->perm, perm.data(), (size_t)V * 4, hipMemcpyHostToDevice));
       HIP_CHECK(hipMemcpy(g->inv, g->h_inv.data(), (size_t)V * 4, hipMemcpyHostToDevice));
     }
+    // Degree order puts the vertices without edges last; on a symmetric graph
+    // nothing reaches them, so full compaction scans stop before them.
+    g->scan_v = V;
+    if (!perm.empty() && g->symmetric)
+      while (g->scan_v > 0 && h.off[g->scan_v] == h.off[g->scan_v - 1]) --g->scan_v;
     {
       std::vector<int32_t> lo;
       std::vector<int32_t> hi;
@@ -3227,6 +3239,7 @@ cgp_status cgp_graph_create_csr_device(int32_t V, int64_t E, const int32_t* off,
     cgp_graph* g = own.g;
     g->borrowed = true;
     g->V = V;
+    g->scan_v = V;
     g->E = E;
     g->off = const_cast<int32_t*>(off);
     g->idx = const_cast<int32_t*>(idx);

@@ -57,6 +57,9 @@ struct cgp_graph {
   int32_t* inv = nullptr;
   std::vector<int32_t> h_inv;
   int32_t internal(int32_t v) const { return h_inv.empty() ? v : h_inv[(size_t)v]; }
+  // Vertices a full compaction scan visits: all of them, or on a symmetric
+  // graph stored in degree order those with edges (a prefix).
+  int32_t scan_v = 0;
 };
 
 namespace cgpi {
@@ -210,6 +213,9 @@ struct cgp_engine {
     P.iters = (cgpk::IterRec*)iters.p;
     P.bin_small = opts.bin_small;
     P.bin_large = opts.bin_large;
+    // Serving seeds queries into running iterations, where a full scan must
+    // reach a source without edges too.
+    P.scan_v = serve ? g->V : g->scan_v;
     return P;
   }
 

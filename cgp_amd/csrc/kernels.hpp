@@ -206,6 +206,10 @@ struct KP {
   IterRec* iters;
   int32_t bin_small;
   int32_t bin_large;
+  // Vertices a full compaction scan visits: [0, scan_v).  On a symmetric
+  // graph stored in degree order the vertices without edges come last, and
+  // only seeding (a touched-list scan) can give them frontier bits.
+  int32_t scan_v;
 };
 
 // Profile counter slots.

@@ -36,22 +36,31 @@ struct SnapArgs {
 };
 
 // For each listed slot (word, lane) -> output row: copies the slot's value
-// column into out[row * V + v] (when out is set), then resets the column to
-// INF and clears the slot's visited bit, leaving the slot ready for its next
-// query.
-__global__ __launch_bounds__(BLOCK) void k_snapshot_reset(u32* vals, u64* visited, const int32_t* inv, int32_t V,
-                                                          int W, int rshift, SnapArgs a, u32* out) {
+// column into out[row * V + v], v in the caller's numbering (the value rows
+// are indexed by the internal vertex inv[v]; inv null: the same).
+__global__ __launch_bounds__(BLOCK) void k_snapshot(const u32* vals, const int32_t* inv, int32_t V, int rshift,
+                                                    SnapArgs a, u32* out) {
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
-    // The caller's vertex v is u inside the graph (inv null: the same).
     const size_t u = (size_t)(inv ? inv[v] : (int32_t)v);
     for (int k = 0; k < a.n; ++k) {
       const u32 e = a.ent[k];
-      const size_t at = (((size_t)(e >> 24) * (size_t)V + u) << rshift) + ((e >> 16) & 0xFFu);
-      if (out) out[(size_t)(e & 0xFFFFu) * V + v] = vals[at];
-      vals[at] = INF;
+      out[(size_t)(e & 0xFFFFu) * V + v] = vals[(((size_t)(e >> 24) * (size_t)V + u) << rshift) + ((e >> 16) & 0xFFu)];
+    }
+  }
+}
+
+// Resets the listed slots' value columns to INF and clears their visited
+// bits, leaving the slots ready for their next queries; in internal vertex
+// order, so the row and record writes stream.
+__global__ __launch_bounds__(BLOCK) void k_slot_reset(u32* vals, u64* visited, int32_t V, int W, int rshift,
+                                                      SnapArgs a) {
+  for (int64_t u = (int64_t)blockIdx.x * BLOCK + threadIdx.x; u < V; u += (int64_t)gridDim.x * BLOCK) {
+    for (int k = 0; k < a.n; ++k) {
+      const u32 e = a.ent[k];
+      vals[(((size_t)(e >> 24) * (size_t)V + (size_t)u) << rshift) + ((e >> 16) & 0xFFu)] = INF;
     }
     for (int w = 0; w < W; ++w)
-      if (a.clear[w]) visited[u * (2 * W) + w] &= ~a.clear[w];  // visited half of the record
+      if (a.clear[w]) visited[(size_t)u * (2 * W) + w] &= ~a.clear[w];  // visited half of the record
   }
 }
 
@@ -284,9 +293,11 @@ void retire(cgp_engine* en, ServeState* ss, const std::vector<int>& conv) {
       a.ent[i] = ((u32)(slot >> 6) << 24) | ((u32)(slot & 63) << 16) | (u32)(c0 + i);
       a.clear[slot >> 6] |= 1ull << (slot & 63);
     }
-    hipLaunchKernelGGL(k_snapshot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
-                       (u64*)en->visited.p, (const int32_t*)en->g->inv, V, ss->W, ss->rshift, a,
-                       staging ? (u32*)staging->p : nullptr);
+    if (staging)
+      hipLaunchKernelGGL(k_snapshot, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (const u32*)en->vals.p,
+                         (const int32_t*)en->g->inv, V, ss->rshift, a, (u32*)staging->p);
+    hipLaunchKernelGGL(k_slot_reset, dim3(en->cus * 8), dim3(BLOCK), 0, en->s, (u32*)en->vals.p,
+                       (u64*)en->visited.p, V, ss->W, ss->rshift, a);
     HIP_CHECK(hipGetLastError());
   }
   std::vector<cgp_completion> comps((size_t)k);

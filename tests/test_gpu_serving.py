@@ -196,3 +196,41 @@ def test_autotune_then_parity(gpu):
     out = en.run_batch(kinds, src2)[0]
     for i in range(0, 140, 9):
         assert np.array_equal(out[i], O.expected(og, 0 if kinds[i] == BFS else 1, src2[i]))
+
+
+def test_isolated_sources_on_a_renumbered_graph(gpu):
+    # RMAT-12 is skewed enough to be stored in degree order, with its
+    # vertices without edges last, where full compaction scans of batch runs
+    # stop.  A source without edges must still get depth / distance 0 in a
+    # batch and when serving admits it into running (dense-mode) iterations.
+    h = cgp_amd.HostCSR.rmat(12, 16, seed=41, wmin=1, wmax=50)
+    deg = np.diff(np.asarray(h.offsets))
+    assert deg.max() >= 32 * h.E / h.V
+    iso = [int(x) for x in np.flatnonzero(deg == 0)[:3]]
+    assert len(iso) == 3
+    g = cgp_amd.Graph.from_host(h)
+    og = O.G.of(h)
+    src = [int(x) for x in h.sample_sources(60, seed=3)]
+    kinds = [BFS] * 30 + [SSSP] * 30 + [BFS, SSSP, BFS]
+    sources = src + iso
+    out = cgp_amd.Engine(g).run_batch(kinds, sources)[0]
+    for i in range(len(kinds)):
+        assert np.array_equal(out[i], O.expected(og, kinds[i], sources[i])), i
+    en = cgp_amd.Engine(g)
+    en.serve_start(max_slots=128)
+    for i in range(60):
+        en.serve_submit(kinds[i], sources[i], i, 0.0)
+    done = {}
+    for c in en.serve_step(max_iterations=2):
+        done[c["ticket"]] = c
+    for j, s in enumerate(iso):
+        en.serve_submit(kinds[60 + j], s, 60 + j, 0.0)
+    steps = 0
+    while len(done) < 63:
+        steps += 1
+        assert steps < 5000, "serving made no progress"
+        for c in en.serve_step(max_iterations=1):
+            done[c["ticket"]] = c
+    en.serve_stop()
+    for i in range(63):
+        assert np.array_equal(done[i]["result"], O.expected(og, kinds[i], sources[i])), i
