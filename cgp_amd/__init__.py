@@ -171,18 +171,21 @@ class Graph:
         self.symmetric = bool(lib.cgp_graph_is_symmetric(self._h))
 
     @classmethod
-    def from_host(cls, h):
+    def from_host(cls, h, renumber=None):
+        """Device graph of a host CSR.  renumber: None (the library decides:
+        large skewed graphs are stored in descending-degree order), True or
+        False; results are always in the caller's numbering."""
         out = C.c_void_p()
-        check(lib.cgp_graph_from_host(h._h, C.byref(out)))
+        check(lib.cgp_graph_from_host_ex(h._h, _renumber(renumber), C.byref(out)))
         return cls(out.value)
 
     @classmethod
-    def from_csr(cls, offsets, indices, weights=None):
+    def from_csr(cls, offsets, indices, weights=None, renumber=None):
         o, op = _i32(offsets)
         i, ip = _i32(indices)
         w, wp = _u32(weights)
         out = C.c_void_p()
-        check(lib.cgp_graph_create_csr(len(o) - 1, len(i), op, ip, wp, C.byref(out)))
+        check(lib.cgp_graph_create_csr_ex(len(o) - 1, len(i), op, ip, wp, _renumber(renumber), C.byref(out)))
         return cls(out.value)
 
     @classmethod
@@ -203,6 +206,10 @@ class Graph:
 
     def __del__(self):
         self.close()
+
+
+def _renumber(r):
+    return _lib.RENUMBER_AUTO if r is None else (_lib.RENUMBER_ON if r else _lib.RENUMBER_OFF)
 
 
 class DeviceBuffer:

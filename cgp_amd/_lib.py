@@ -27,6 +27,7 @@ INF = 0xFFFFFFFF
 FORMAT_GR, FORMAT_MATRIX_MARKET, FORMAT_CSR_DIRECTORY = 0, 1, 2
 LOAD_UNIT_WEIGHTS, LOAD_ROUND_WEIGHTS = 1, 2
 REPORT_JSON, REPORT_CSV = 0, 1
+RENUMBER_AUTO, RENUMBER_ON, RENUMBER_OFF = 0, 1, 2
 
 
 class cgp_query(C.Structure):
@@ -167,6 +168,8 @@ SIGNATURES = {
     "cgp_poisson_arrivals": (_st, [C.c_int32, C.c_double, C.c_uint64, _dp]),
     "cgp_graph_create_csr": (_st, [C.c_int32, C.c_int64, _i32p, _i32p, _u32p, _pp]),
     "cgp_graph_from_host": (_st, [_p, _pp]),
+    "cgp_graph_create_csr_ex": (_st, [C.c_int32, C.c_int64, _i32p, _i32p, _u32p, C.c_int32, _pp]),
+    "cgp_graph_from_host_ex": (_st, [_p, C.c_int32, _pp]),
     "cgp_graph_create_csr_device": (_st, [C.c_int32, C.c_int64, _p, _p, _p, _pp]),
     "cgp_graph_destroy": (None, [_p]),
     "cgp_graph_num_vertices": (C.c_int32, [_p]),

@@ -437,14 +437,21 @@ __device__ __forceinline__ int wave_upper_bound(u32 incl, u32 f) {
 // Phase S: a wave takes up to 64 low-degree frontier vertices at once; the
 // lanes load their metadata in parallel, a wave prefix sum over the degrees
 // flattens their edge lists, and each step relaxes 64 consecutive flattened
-// edges, one per lane (load-balanced lane-per-edge).
+// edges, one per lane (load-balanced lane-per-edge).  Group g of the n list
+// entries is eight runs of eight consecutive entries, runs g, g + ng, ...,
+// g + 7 ng of the list's n/8 runs (ng groups): neighbouring entries keep
+// their shared metadata lines, while a group mixes entries from across the
+// list, whose order follows vertex ids and so, on degree-ordered graphs,
+// degree (groups of 64 equal degrees made single waves walk the longest
+// edge lists).
 template <int W, bool SPARSE>
-__device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 base, u32 n,
+__device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 g, u32 ng, u32 n,
                                                      bool bfs_on, int dense_t, int rshift, u32 stamp,
                                                      u32 thr, Tally& t) {
   const int lane = lane_id();
-  const bool have = base + (u32)lane < n;
-  const int32_t u = have ? list[base + lane] : 0;
+  const u32 e = (g + (u32)(lane >> 3) * ng) * 8u + (u32)(lane & 7);
+  const bool have = e < n;
+  const int32_t u = have ? list[e] : 0;
   u64 bm[W], sm[W], far[W];
   bool any_s = false, any_b = false;
   if (have) load_masks<W>(P, u, bfs_on, dense_t, thr, rshift, bm, sm, far, any_s, t);
@@ -1176,11 +1183,12 @@ __device__ __forceinline__ void advance_push_body(const KP& P, u32* s_item, Tall
       expand_lanes<W, SPARSE>(P, u, e0, e1, e0 + lane, 64, bfs_on, dense_t, rshift, stamp, thr, lane == 0, t);
     }
   }
-  // Phase S.
+  // Phase S: groups of 64 entries (expand_flat_vertices).
   const u32 nS = C->bcount[p][0].v;
-  for (ShardFetch sf; nS && sf.next(C->shard[p][SH_S], nS, 64u * FETCH_S, lo, hi);) {
-    for (u32 base = lo; base < hi; base += 64u)
-      expand_flat_vertices<W, SPARSE>(P, P.small[p], base, hi, bfs_on, dense_t, rshift, stamp, thr, t);
+  const u32 ngS = (nS + 63u) / 64u;
+  for (ShardFetch sf; nS && sf.next(C->shard[p][SH_S], ngS, FETCH_S, lo, hi);) {
+    for (u32 g = lo; g < hi; ++g)
+      expand_flat_vertices<W, SPARSE>(P, P.small[p], g, ngS, nS, bfs_on, dense_t, rshift, stamp, thr, t);
   }
   // Phase P: high-degree vertices one wave each (64 neighbours per step,
   // early exit on a wave-wide OR), then the rest one lane each.
@@ -2308,15 +2316,18 @@ void launch_dense_u(hipStream_t s, dim3 grid, const KP& P, int tab) {
 
 // Optional advance launches of the current slot configuration: bit 0, the
 // lane-per-query kernels (some word has SSSP slots); bit 1, the narrow
-// kernel (some word is narrow); bit 2, the SSSP pull kernel (SSSP pull
-// steps enabled).  The filter routes no item to a kernel that
-// is not launched, and a BFS-only batch saves three launches per iteration.
+// kernel (some word is narrow now, or was when the last filter binned the
+// frontier this advance expands: serving admission can end the narrow
+// regime between a filter and the next advance); bit 2, the SSSP pull
+// kernel (SSSP pull steps enabled).  The filter routes no item to a kernel
+// that is not launched, and a BFS-only batch saves three launches per
+// iteration.
 int launch_set(const cgp_engine* en) {
   const Cfg& c = *en->h_cfg;
   int ls = 0;
   for (int w = 0; w < MAXW; ++w)
     if (c.val_bits[w]) ls |= 1;
-  if (c.narrow_words) ls |= 2;
+  if (c.narrow_words || en->narrow_binned) ls |= 2;
   if (c.sssp_pull_policy) ls |= 4;
   return ls;
 }
@@ -2371,6 +2382,7 @@ void launch_iteration(cgp_engine* en, const KP& P) {
   }
   hipLaunchKernelGGL((k_filter<W>), dim3(en->grid_filter()), dim3(BLOCK), 0, s, P);
   HIP_CHECK(hipGetLastError());
+  en->narrow_binned = en->h_cfg->narrow_words;
 }
 
 template <int W>
@@ -2534,9 +2546,6 @@ void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) 
   HIP_CHECK(hipGetLastError());
 }
 
-// Runs the frontier iterations of one pass to convergence: replays of a
-// captured K-iteration graph (or direct launches), one host readback of the
-// control block per replay.
 // Depth bit planes [plane_count, need) of the running pass: allocated (a
 // word of V bits per mask word), zeroed over the pass's BFS words, and
 // published in the slot configuration (on the device too when `live`: the
@@ -2575,6 +2584,9 @@ static void add_depth_planes(cgp_engine* en, int W, int need, bool live) {
 
 static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
+// Runs the frontier iterations of one pass to convergence: replays of a
+// captured K-iteration graph (or direct launches), one host readback of the
+// control block per replay.
 static void run_iterations(cgp_engine* en, int W, bool no_graph) {
   const KP P = en->kp();
   const bool use_graph = en->opts.use_hip_graph != 0 && !no_graph;
@@ -2743,6 +2755,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
       }
     }
     cfg.narrow_words = rshift >= 4 && cfg.delta == 0 && en->opts.dense_narrow ? narrow_mask(cfg, W) : 0u;
+    en->narrow_binned = cfg.narrow_words;  // fixed for the pass, its seeding filter included
     // Bottom-up steps pay off only when few BFS queries share the words: with
     // many, the missing bits of a vertex are rarely all found early.  Hybrid
     // mode pulls only for batches of at most pull_max_queries BFS queries
@@ -3125,8 +3138,15 @@ cgp_status cgp_copy_device(void* dst, const void* src, uint64_t bytes) {
 
 cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,
                                 const uint32_t* wt, cgp_graph** out) {
+  return cgp_graph_create_csr_ex(V, E, off, idx, wt, CGP_RENUMBER_AUTO, out);
+}
+
+cgp_status cgp_graph_create_csr_ex(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,
+                                   const uint32_t* wt, int32_t renumber, cgp_graph** out) {
   CGP_TRY({
     if (!out) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: null output");
+    if (renumber < CGP_RENUMBER_AUTO || renumber > CGP_RENUMBER_OFF)
+      cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: renumber must be CGP_RENUMBER_AUTO, _ON or _OFF");
     *out = nullptr;
     if (V < 0 || E < 0 || E >= INT32_MAX) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: bad sizes");
     if (!off || (E > 0 && !idx)) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr: null CSR array");
@@ -3153,17 +3173,21 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
       for (int64_t e = 0; e < E; ++e) sum += wt[e];
       g->mean_weight = sum / (double)E;
     }
-    // Skewed graphs are stored with their vertices renumbered by descending
-    // degree: the hubs that most edges reach get neighbouring value rows and
-    // records, which share DRAM pages, TLB entries and cache sets (RMAT-22,
-    // 160 BFS + 80 SSSP + 16 WCC: 54.8 -> 50.2 ms per batch).  Graphs whose
-    // largest degree is within 32x the mean (meshes, grids) keep their
-    // numbering, whose neighbours are already close.  Results, sources and
-    // WCC labels stay in the caller's numbering.
+    // Large skewed graphs are stored with their vertices renumbered by
+    // descending degree: the hubs that most edges reach get neighbouring value
+    // rows and records, which share DRAM pages, TLB entries and cache sets
+    // (RMAT-22, 160 BFS + 80 SSSP + 16 WCC: 54.8 -> 50.2 ms per batch).  Below
+    // 2^21 vertices a word's value rows (256 bytes a vertex) fit the on-chip
+    // cache, and the renumbering measured slower (RMAT-20 64 SSSP: 9.0 -> 9.5
+    // ms; RMAT-16 32 BFS: 0.79 -> 1.02 ms); graphs whose largest degree is
+    // within 32x the mean (meshes, grids) keep their numbering too, their
+    // neighbours being close already.  Results, sources and WCC labels stay
+    // in the caller's numbering.
     int32_t maxd = 0;
     for (int32_t v = 0; v < V; ++v) maxd = std::max(maxd, off[v + 1] - off[v]);
     std::vector<int32_t> perm;
-    if (V > 0 && (double)maxd >= 32.0 * std::max(1.0, (double)E / (double)V))
+    const bool auto_on = V >= (1 << 21) && (double)maxd >= 32.0 * std::max(1.0, (double)E / (double)V);
+    if (V > 0 && (renumber == CGP_RENUMBER_ON || (renumber == CGP_RENUMBER_AUTO && auto_on)))
       h = cgpi::relabel_by_degree(h, perm, g->h_inv);
     HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
     HIP_CHECK(hipMalloc((void**)&g->idx, (size_t)E * 4 + 4));
@@ -3211,12 +3235,16 @@ cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const 
 }
 
 cgp_status cgp_graph_from_host(const cgp_host_csr* h, cgp_graph** out) {
+  return cgp_graph_from_host_ex(h, CGP_RENUMBER_AUTO, out);
+}
+
+cgp_status cgp_graph_from_host_ex(const cgp_host_csr* h, int32_t renumber, cgp_graph** out) {
   if (!h) {
     cgpi::set_last_error("cgp_graph_from_host: null host graph");
     return CGP_EINVAL;
   }
-  return cgp_graph_create_csr(h->g.V, h->g.E, h->g.off.data(), h->g.idx.data(),
-                              h->g.wt.empty() ? nullptr : h->g.wt.data(), out);
+  return cgp_graph_create_csr_ex(h->g.V, h->g.E, h->g.off.data(), h->g.idx.data(),
+                                 h->g.wt.empty() ? nullptr : h->g.wt.data(), renumber, out);
 }
 
 cgp_status cgp_graph_create_csr_device(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,

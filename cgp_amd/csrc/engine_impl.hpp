@@ -153,6 +153,9 @@ struct cgp_engine {
   int32_t* h_stage = nullptr;  // pinned staging (slot sources, qmap, wcc rows)
   size_t h_stage_cap = 0;
   u32 stamp_epoch = 1;
+  // Narrow words of the configuration the last launched filter ran with: its
+  // narrow-word items wait for the next advance's narrow kernel.
+  u32 narrow_binned = 0;
   bool next_dirty = false;
   bool busy = false;
   std::mutex mu;

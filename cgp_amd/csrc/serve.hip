@@ -431,6 +431,7 @@ cgp_status cgp_serve_start(cgp_engine* en, const cgp_serve_opts* so) {
     en->next_dirty = false;
     Cfg& cfg = *en->h_cfg;
     std::memset(&cfg, 0, sizeof(Cfg));
+    en->narrow_binned = 0;
     cfg.rshift = rshift;
     cfg.W = W;
     cfg.pull_policy = 0;

@@ -234,6 +234,18 @@ cgp_status cgp_graph_create_csr(int32_t n_vertices, int64_t n_edges, const int32
                                 const int32_t* col_indices, const uint32_t* weights,
                                 cgp_graph** out);
 cgp_status cgp_graph_from_host(const cgp_host_csr* g, cgp_graph** out);
+/* As cgp_graph_create_csr / cgp_graph_from_host, with the internal vertex numbering chosen
+ * by `renumber`: CGP_RENUMBER_AUTO (what the two entries above do: renumber by descending
+ * degree when the graph has at least 2^21 vertices and its largest degree is at least 32x
+ * the mean), CGP_RENUMBER_ON (always), CGP_RENUMBER_OFF (keep the caller's numbering).  The
+ * numbering is internal: sources, results and labels are always in the caller's numbering. */
+#define CGP_RENUMBER_AUTO 0
+#define CGP_RENUMBER_ON 1
+#define CGP_RENUMBER_OFF 2
+cgp_status cgp_graph_create_csr_ex(int32_t n_vertices, int64_t n_edges, const int32_t* row_offsets,
+                                   const int32_t* col_indices, const uint32_t* weights, int32_t renumber,
+                                   cgp_graph** out);
+cgp_status cgp_graph_from_host_ex(const cgp_host_csr* g, int32_t renumber, cgp_graph** out);
 /* Non-owning graph over caller-owned device (HBM) arrays; replaces the non-owning
  * csr_graph_view (puercgp/include/puercgp/backend/csr_graph.hxx:9-20).  row_offsets (V+1),
  * col_indices (E) and weights (E, or NULL for unit weights) are device pointers that must

@@ -122,15 +122,17 @@ class host_csr {
 // Device-resident CSR; the graph concept's accessors for host code.
 class graph {
  public:
-  explicit graph(const host_csr& h) {
+  // renumber: CGP_RENUMBER_AUTO / _ON / _OFF, the internal vertex order (results
+  // are always in the caller's numbering).
+  explicit graph(const host_csr& h, int32_t renumber = CGP_RENUMBER_AUTO) {
     cgp_graph* g = nullptr;
-    check(cgp_graph_from_host(h.handle(), &g));
+    check(cgp_graph_from_host_ex(h.handle(), renumber, &g));
     g_.reset(g);
   }
   graph(int32_t V, int64_t E, const int32_t* row_offsets, const int32_t* column_indices,
-        const uint32_t* weights = nullptr) {
+        const uint32_t* weights = nullptr, int32_t renumber = CGP_RENUMBER_AUTO) {
     cgp_graph* g = nullptr;
-    check(cgp_graph_create_csr(V, E, row_offsets, column_indices, weights, &g));
+    check(cgp_graph_create_csr_ex(V, E, row_offsets, column_indices, weights, renumber, &g));
     g_.reset(g);
   }
   // Non-owning view over the caller's device (HBM) arrays, as csr_graph_view
