@@ -438,18 +438,22 @@ __device__ __forceinline__ int wave_upper_bound(u32 incl, u32 f) {
 // lanes load their metadata in parallel, a wave prefix sum over the degrees
 // flattens their edge lists, and each step relaxes 64 consecutive flattened
 // edges, one per lane (load-balanced lane-per-edge).  Group g of the n list
-// entries is eight runs of eight consecutive entries, runs g, g + ng, ...,
-// g + 7 ng of the list's n/8 runs (ng groups): neighbouring entries keep
-// their shared metadata lines, while a group mixes entries from across the
-// list, whose order follows vertex ids and so, on degree-ordered graphs,
-// degree (groups of 64 equal degrees made single waves walk the longest
-// edge lists).
+// entries (ng groups) is entries 64g..64g+63, or, on skewed graphs
+// (mix_small), eight runs of eight consecutive entries, runs g, g + ng, ...,
+// g + 7 ng of the list's n/8 runs.  The list follows vertex ids, and on
+// skewed graphs ids follow degree (degree-ordered storage, or an RMAT
+// generator's own bias), so groups of 64 near-equal degrees made single waves
+// walk the longest edge lists; mixed runs of eight keep neighbouring entries'
+// shared metadata lines (RMAT-22 mixed 256: 50.1 -> 47.2 ms per batch).  On
+// grids and meshes consecutive ids are neighbours whose rows share cache
+// lines, and mixing measured slower (4096^2 grid, 512 SSSP: push 1067 -> 1195
+// ms per batch).
 template <int W, bool SPARSE>
 __device__ __forceinline__ void expand_flat_vertices(const KP& P, const int32_t* list, u32 g, u32 ng, u32 n,
                                                      bool bfs_on, int dense_t, int rshift, u32 stamp,
                                                      u32 thr, Tally& t) {
   const int lane = lane_id();
-  const u32 e = (g + (u32)(lane >> 3) * ng) * 8u + (u32)(lane & 7);
+  const u32 e = P.mix_small ? (g + (u32)(lane >> 3) * ng) * 8u + (u32)(lane & 7) : g * 64u + (u32)lane;
   const bool have = e < n;
   const int32_t u = have ? list[e] : 0;
   u64 bm[W], sm[W], far[W];
@@ -2070,11 +2074,19 @@ __global__ __launch_bounds__(BLOCK) void k_fill_rows(const int32_t* off, int32_t
 // rows: a binary search in the target's row).
 enum : u32 { CSR_BAD_OFFSETS = 1, CSR_BAD_INDEX = 2, CSR_UNSORTED = 4, CSR_ASYMMETRIC = 8, CSR_ASYM_WEIGHTS = 16 };
 
-__global__ __launch_bounds__(BLOCK) void k_csr_check_offsets(const int32_t* off, int32_t V, u32* flags) {
+// Flags decreasing offsets and takes the largest degree (into *maxdeg).
+__global__ __launch_bounds__(BLOCK) void k_csr_check_offsets(const int32_t* off, int32_t V, u32* flags,
+                                                             u32* maxdeg) {
   bool bad = false;
-  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK)
-    bad |= off[v + 1] < off[v];
+  int32_t m = 0;
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < V; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t d = off[v + 1] - off[v];
+    bad |= d < 0;
+    m = max(m, d);
+  }
   if (__ballot(bad) && lane_id() == 0) atomicOr(flags, (u32)CSR_BAD_OFFSETS);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if (lane_id() == 0 && m > 0) atomicMax(maxdeg, (u32)m);
 }
 
 // Edge-parallel over slots whose source row `rows` is known (offsets valid).
@@ -3136,6 +3148,11 @@ cgp_status cgp_copy_device(void* dst, const void* src, uint64_t bytes) {
   });
 }
 
+// Largest degree at least 32x the mean (power-law graphs; not meshes or grids).
+static bool is_skewed(int32_t V, int64_t E, int32_t maxd) {
+  return V > 0 && (double)maxd >= 32.0 * std::max(1.0, (double)E / (double)V);
+}
+
 cgp_status cgp_graph_create_csr(int32_t V, int64_t E, const int32_t* off, const int32_t* idx,
                                 const uint32_t* wt, cgp_graph** out) {
   return cgp_graph_create_csr_ex(V, E, off, idx, wt, CGP_RENUMBER_AUTO, out);
@@ -3185,8 +3202,9 @@ cgp_status cgp_graph_create_csr_ex(int32_t V, int64_t E, const int32_t* off, con
     // in the caller's numbering.
     int32_t maxd = 0;
     for (int32_t v = 0; v < V; ++v) maxd = std::max(maxd, off[v + 1] - off[v]);
+    g->skewed = is_skewed(V, E, maxd);
     std::vector<int32_t> perm;
-    const bool auto_on = V >= (1 << 21) && (double)maxd >= 32.0 * std::max(1.0, (double)E / (double)V);
+    const bool auto_on = V >= (1 << 21) && g->skewed;
     if (V > 0 && (renumber == CGP_RENUMBER_ON || (renumber == CGP_RENUMBER_AUTO && auto_on)))
       h = cgpi::relabel_by_degree(h, perm, g->h_inv);
     HIP_CHECK(hipMalloc((void**)&g->off, ((size_t)V + 1) * 4));
@@ -3292,8 +3310,12 @@ cgp_status cgp_graph_create_csr_device(int32_t V, int64_t E, const int32_t* off,
       HIP_CHECK(hipMemcpy(&f, flags, 4, hipMemcpyDeviceToHost));
       return f;
     };
-    if (V > 0) hipLaunchKernelGGL(k_csr_check_offsets, gb, dim3(BLOCK), 0, 0, off, V, flags);
+    u32* maxdeg = own.d_tmp + 6;
+    if (V > 0) hipLaunchKernelGGL(k_csr_check_offsets, gb, dim3(BLOCK), 0, 0, off, V, flags, maxdeg);
     if (read_flags() & CSR_BAD_OFFSETS) cgpi::fail(CGP_EINVAL, "cgp_graph_create_csr_device: offsets not monotone");
+    u32 maxd = 0;
+    HIP_CHECK(hipMemcpy(&maxd, maxdeg, 4, hipMemcpyDeviceToHost));
+    g->skewed = is_skewed(V, E, (int32_t)maxd);
     HIP_CHECK(hipMalloc((void**)&g->rows, (size_t)E * 4 + 4));
     if (V > 0) hipLaunchKernelGGL(k_fill_rows, gb, dim3(BLOCK), 0, 0, off, V, g->rows);
     if (E > 0) hipLaunchKernelGGL(k_csr_check_edges, gb, dim3(BLOCK), 0, 0, (const int32_t*)g->rows, idx, E, V, flags);

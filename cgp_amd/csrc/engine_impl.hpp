@@ -37,6 +37,7 @@ struct cgp_graph {
   int64_t E = 0;
   bool symmetric = false;
   bool weighted = false;
+  bool skewed = false;  // largest degree at least 32x the mean
   bool borrowed = false;  // off / idx / wt belong to the caller (cgp_graph_create_csr_device)
   bool sym_weights = false;  // symmetric, and every edge's reverse has the same weight (pull gathers)
   double mean_weight = 1.0;  // over all edges (1 when unweighted)
@@ -219,6 +220,7 @@ struct cgp_engine {
     // Serving seeds queries into running iterations, where a full scan must
     // reach a source without edges too.
     P.scan_v = serve ? g->V : g->scan_v;
+    P.mix_small = g->skewed ? 1 : 0;
     return P;
   }
 

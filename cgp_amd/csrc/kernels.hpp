@@ -210,6 +210,9 @@ struct KP {
   // graph stored in degree order the vertices without edges come last, and
   // only seeding (a touched-list scan) can give them frontier bits.
   int32_t scan_v;
+  // Nonzero on skewed graphs: phase S groups mix entries from across the
+  // small-degree list instead of taking 64 consecutive ones.
+  int32_t mix_small;
 };
 
 // Profile counter slots.
