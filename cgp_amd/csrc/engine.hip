@@ -2541,14 +2541,17 @@ void compute_wcc(cgp_engine* en) {
 
 // Writes the labels of compute_wcc's forest into nrows output rows (d_rows
 // null: one row at out).  On renumbered graphs the smallest caller id of each
-// component is found first, in the touched list's buffer (free between
-// iterations).
-void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out) {
+// component is found first, in `minid` or, when that is null, in the touched
+// list's buffer (free once a batch pass has finished its iterations).
+void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out, int32_t* minid) {
   const cgp_graph* g = en->g;
-  int32_t* minid = nullptr;
-  if (g->inv) {
-    en->touched.ensure((size_t)g->V * 4 + 4);
-    minid = (int32_t*)en->touched.p;
+  if (!g->inv) {
+    minid = nullptr;
+  } else {
+    if (!minid) {
+      en->touched.ensure((size_t)g->V * 4 + 4);
+      minid = (int32_t*)en->touched.p;
+    }
     fill_u32(en->s, minid, (size_t)g->V, 0x7FFFFFFFu, en->cus);
     hipLaunchKernelGGL(k_wcc_minid, dim3(en->cus * 16), dim3(BLOCK), 0, en->s, (const int32_t*)en->parent.p,
                        (const int32_t*)g->inv, g->V, minid);

@@ -335,7 +335,10 @@ void engine_run_batch_static(cgp_engine* en, const cgp_query* queries, int32_t n
 // Computes min-id component labels into en->parent (hooking + pointer jumping).
 void compute_wcc(cgp_engine* en);
 // Writes the labels into rows rows_out[0..nrows) of out (query-major, V per row).
-void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out);
+// On renumbered graphs `minid` (V int32) holds the per-root smallest caller
+// ids; null takes the touched list's buffer, which only a finished batch
+// pass may do (a serving session can hold seeds there between steps).
+void write_wcc_rows(cgp_engine* en, const int32_t* d_rows, int nrows, u32* out, int32_t* minid = nullptr);
 // Serving teardown for cgp_engine_destroy (serve.hip).
 void serve_release_engine(cgp_engine* en);
 

@@ -98,6 +98,10 @@ struct ServeState {
   std::vector<u32> wcc_host;
   cgpi::DevBuf d_src;  // seed sources
   cgpi::DevBuf wcc_lab;  // WCC labels of the session's graph (caller numbering)
+  // Per-root smallest caller ids while the labels are written (renumbered
+  // graphs): the touched list may hold seeds of queries admitted at the end
+  // of the previous step.
+  cgpi::DevBuf wcc_minid;
   // The pinned staging that admissions upload from (slot configuration and
   // seed sources) is rewritten only after the previous upload has run.
   hipEvent_t staged = nullptr;
@@ -166,10 +170,16 @@ void admit(cgp_engine* en, ServeState* ss) {
       if (!ss->wcc_ready) {
         compute_wcc(en);
         ss->wcc_lab.ensure((size_t)V * 4 + 4);
-        write_wcc_rows(en, nullptr, 1, (u32*)ss->wcc_lab.p);
+        int32_t* minid = nullptr;
+        if (en->g->inv) {
+          ss->wcc_minid.ensure((size_t)V * 4 + 4);
+          minid = (int32_t*)ss->wcc_minid.p;
+        }
+        write_wcc_rows(en, nullptr, 1, (u32*)ss->wcc_lab.p, minid);
         ss->wcc_host.resize((size_t)V);
         HIP_CHECK(hipMemcpyAsync(ss->wcc_host.data(), ss->wcc_lab.p, (size_t)V * 4, hipMemcpyDeviceToHost, en->s));
         HIP_CHECK(hipStreamSynchronize(en->s));
+        ss->wcc_minid.release();
         ss->wcc_ready = true;
       }
       cgp_completion c{};
@@ -356,6 +366,7 @@ void serve_free(cgp_engine* en) {
   }
   ss->d_src.release();
   ss->wcc_lab.release();
+  ss->wcc_minid.release();
   // Stamps used by the session must not recur in later runs; slots may hold
   // bits in next[] when stopped mid-run.
   en->stamp_epoch += ss->iter + 2;

@@ -113,3 +113,41 @@ def test_serving_on_a_renumbered_graph(gpu):
     for i in range(len(src)):
         assert np.array_equal(done[i]["result"], O.expected(og, kinds[i], src[i])), i
     assert np.array_equal(done[1000]["result"], O.wcc_uf(og))
+
+
+def test_wcc_admitted_while_seeds_wait(gpu):
+    # Four queries from sources without edges converge in the second
+    # iteration; the same step then admits the four queued ones, whose seeds
+    # wait in the touched list for the next iteration.  A WCC request admitted
+    # at the start of the next step computes its labels on the renumbered
+    # graph (per-root smallest caller ids in a scratch array) and must leave
+    # those seeds in place.
+    h = cgp_amd.HostCSR.rmat(12, 16, seed=41, wmin=1, wmax=50)
+    deg = np.diff(np.asarray(h.offsets))
+    iso = [int(x) for x in np.flatnonzero(deg == 0)[:4]]
+    assert len(iso) == 4
+    g = cgp_amd.Graph.from_host(h, renumber=True)
+    og = O.G.of(h)
+    src = iso + [int(x) for x in h.sample_sources(4, seed=9)]
+    kinds = [BFS, SSSP, BFS, SSSP, BFS, SSSP, SSSP, BFS]
+    en = cgp_amd.Engine(g)
+    en.serve_start(max_slots=4)
+    for i in range(8):
+        en.serve_submit(kinds[i], src[i], i, 0.0)
+    done = {}
+    for _ in range(2):
+        for c in en.serve_step(max_iterations=1):
+            done[c["ticket"]] = c
+    assert en.serve_pending() == (0, 4)  # the last four hold the slots, seeded
+    say("WCC admitted with four seeds waiting")
+    en.serve_submit(WCC, 0, 100, 0.0)
+    steps = 0
+    while len(done) < 9:
+        steps += 1
+        assert steps < 5000, "serving made no progress"
+        for c in en.serve_step(0):
+            done[c["ticket"]] = c
+    en.serve_stop()
+    for i in range(8):
+        assert np.array_equal(done[i]["result"], O.expected(og, kinds[i], src[i])), i
+    assert np.array_equal(done[100]["result"], O.wcc_uf(og))
