@@ -234,8 +234,9 @@ cgp_status cgp_graph_create_csr(int32_t n_vertices, int64_t n_edges, const int32
                                 const int32_t* col_indices, const uint32_t* weights,
                                 cgp_graph** out);
 cgp_status cgp_graph_from_host(const cgp_host_csr* g, cgp_graph** out);
-/* As cgp_graph_create_csr / cgp_graph_from_host, with the internal vertex numbering chosen
- * by `renumber`: CGP_RENUMBER_AUTO (what the two entries above do: renumber by descending
+/* As cgp_graph_create_csr / cgp_graph_from_host (the same csr_graph_storage / graph::build
+ * counterparts, which always store the caller's numbering), with the internal vertex
+ * numbering chosen by `renumber`: CGP_RENUMBER_AUTO (what the two entries above do: renumber by descending
  * degree when the graph has at least 2^21 vertices and its largest degree is at least 32x
  * the mean), CGP_RENUMBER_ON (always), CGP_RENUMBER_OFF (keep the caller's numbering).  The
  * numbering is internal: sources, results and labels are always in the caller's numbering. */
