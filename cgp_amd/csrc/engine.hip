@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -2601,8 +2602,14 @@ static int bit_length(u64 x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 // Runs the frontier iterations of one pass to convergence: replays of a
 // captured K-iteration graph (or direct launches), one host readback of the
-// control block per replay.
-static void run_iterations(cgp_engine* en, int W, bool no_graph) {
+// control block per replay.  `output` (the pass's result kernels) is
+// enqueued behind the replay in which the engine's previous pass finished,
+// ahead of that replay's readback, so a pass that finishes there as well
+// returns with its output already on the stream (true); otherwise the caller
+// runs it again once the pass has finished (those kernels only read the
+// traversal state, so the early copy is overwritten).  This takes the
+// readback's round trip (about 70 us) off the end of a batch.
+static bool run_iterations(cgp_engine* en, int W, bool no_graph, const std::function<void()>& output) {
   const KP P = en->kp();
   const bool use_graph = en->opts.use_hip_graph != 0 && !no_graph;
   const int K = use_graph ? std::max(1, en->opts.graph_iterations) : 1;
@@ -2622,6 +2629,8 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     gx.prof = en->opts.profile;
   }
   u32 iter = 0;  // the seeding compaction leaves iteration 0
+  int replays = 0;
+  bool out_done = false;
   for (;;) {
     // Iteration i writes BFS depth i + 1: planes for every depth of the
     // iterations this launch may run.
@@ -2631,6 +2640,9 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     } else {
       dispatch_iteration(en, W, P);
     }
+    ++replays;
+    out_done = replays == en->spec_replay;
+    if (out_done) output();
     HIP_CHECK(hipMemcpyAsync(en->h_ctl, en->d_ctl, CTL_HOST_BYTES, hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
     const u32 after = en->h_ctl->iter;
@@ -2643,6 +2655,8 @@ static void run_iterations(cgp_engine* en, int W, bool no_graph) {
     if ((int64_t)after > ((int64_t)en->h_ctl->bumps + 1) * ((int64_t)en->g->V + 2))
       cgpi::fail(CGP_EDEVICE, "traversal did not converge within V+2 iterations per threshold step");
   }
+  en->spec_replay = replays;
+  return out_done;
 }
 
 
@@ -2698,6 +2712,26 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   cgp_run_stats st{};
   std::vector<IterRec>& report = en->report;
   report.clear();
+  // Iteration records copied into en->h_rings and not yet decoded: report
+  // slots [first, first + nrec) of one pass, from area offset `at`.
+  struct RingCopy {
+    size_t first, nrec, at;
+    u32 iters, pass;
+  };
+  std::vector<RingCopy> rings;
+  size_t rings_used = 0;
+  // Decodes the copies (their transfers complete: the stream has been
+  // synchronised) and empties the area.
+  const auto decode_rings = [&]() {
+    for (const RingCopy& c : rings)
+      for (size_t k = 0; k < c.nrec; ++k) {
+        const u32 it = c.iters - (u32)c.nrec + (u32)k;
+        report[c.first + k] = en->h_rings[c.at + it % ITER_RING];
+        report[c.first + k].pull |= c.pass << 16;  // pass index in the high half
+      }
+    rings.clear();
+    rings_used = 0;
+  };
   en->rep_seed.clear();
   en->rep_wall_ms = 0;
   if (!no_alloc) {
@@ -2848,7 +2882,27 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     en->next_dirty = false;
     dispatch_seed(en, W, P, (const int32_t*)en->slot_src.p, W * 64);
     en->tick(CAT_INIT, false);
-    run_iterations(en, W, no_alloc);
+    // The pass's output: query-major rows from the value rows, visited records
+    // and depth planes (the planes the replays have added by then).
+    const auto emit_output = [&]() {
+      en->tick(CAT_OUT, true);
+      const int vec = rshift == 6 && (V & 3) == 0 && ((uintptr_t)dout & 15) == 0;
+      OutMasks om{};
+      for (int w = 0; w < Wused; ++w) {
+        om.bfs[w] = cfg.bfs_bits[w];
+        om.val[w] = cfg.val_bits[w];
+      }
+      om.n_planes = cfg.depth_planes ? en->plane_count : 0;
+      om.plane4 = (const u64*)en->plane4.p;
+      for (int b = 4; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
+      hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK),
+                         (size_t)om.n_planes * OUT_VT * sizeof(u64), en->s, (const u32*)en->vals.p,
+                         (const u64*)en->visited.p, (const int32_t*)g->inv, V, W, rshift,
+                         (const int32_t*)en->qmap.p, om, dout, vec);
+      HIP_CHECK(hipGetLastError());
+      en->tick(CAT_OUT, false);
+    };
+    const bool out_done = run_iterations(en, W, no_alloc, emit_output);
     const Ctl& hc = *en->h_ctl;
     st.iterations += (int32_t)hc.iter;
     st.push_iterations += (int32_t)hc.push_iters;
@@ -2857,21 +2911,23 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     if (en->counted()) add_work_bytes(st, hc.prof, W);
     if (!no_alloc) {
       en->rep_seed.push_back({hc.seed_pairs, hc.seed_verts, hc.seed_edges, hc.seed_qedges, hc.t_start});
-      // Per-iteration records of this pass (the ring keeps the last ITER_RING).
+      // Per-iteration records of this pass (the ring keeps the last ITER_RING):
+      // copied on the engine's stream (the legacy stream would wait on, and fail
+      // beside, other threads' engines capturing graphs) into the pinned area,
+      // and decoded after the batch's final synchronisation.
       const size_t nrec = std::min<size_t>(hc.iter, (size_t)ITER_RING);
       const size_t first = report.size();
       report.resize(first + nrec);
       if (nrec) {
-        std::vector<IterRec> ring(nrec);
-        // On the engine's stream: the legacy stream would wait on (and fail
-        // beside) other threads' engines capturing graphs.
-        HIP_CHECK(hipMemcpyAsync(ring.data(), en->iters.p, nrec * sizeof(IterRec), hipMemcpyDeviceToHost, en->s));
-        HIP_CHECK(hipStreamSynchronize(en->s));
-        for (size_t k = 0; k < nrec; ++k) {
-          const u32 it = hc.iter - (u32)nrec + (u32)k;
-          report[first + k] = ring[it % ITER_RING];
-          report[first + k].pull |= (u32)(st.passes - 1) << 16;  // pass index in the high half
+        if (rings_used + nrec > en->h_rings_cap) {
+          HIP_CHECK(hipStreamSynchronize(en->s));
+          decode_rings();
+          en->stage_rings(std::max(nrec, (size_t)256));
         }
+        HIP_CHECK(hipMemcpyAsync(en->h_rings + rings_used, en->iters.p, nrec * sizeof(IterRec),
+                                 hipMemcpyDeviceToHost, en->s));
+        rings.push_back({first, nrec, rings_used, hc.iter, (u32)(st.passes - 1)});
+        rings_used += nrec;
       }
     }
     if (en->timed()) {
@@ -2883,22 +2939,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
     }
     en->stamp_epoch += hc.iter + 2;
     if (hc.max_iter != 0 && hc.iter >= hc.max_iter) en->next_dirty = true;  // truncated run
-    en->tick(CAT_OUT, true);
-    const int vec = rshift == 6 && (V & 3) == 0 && ((uintptr_t)dout & 15) == 0;
-    OutMasks om{};
-    for (int w = 0; w < Wused; ++w) {
-      om.bfs[w] = cfg.bfs_bits[w];
-      om.val[w] = cfg.val_bits[w];
-    }
-    om.n_planes = cfg.depth_planes ? en->plane_count : 0;
-    om.plane4 = (const u64*)en->plane4.p;
-    for (int b = 4; b < om.n_planes; ++b) om.plane[b] = (const u64*)en->plane[b].p;
-    hipLaunchKernelGGL(k_output, dim3((V + OUT_VT - 1) / OUT_VT, Wused), dim3(BLOCK),
-                       (size_t)om.n_planes * OUT_VT * sizeof(u64), en->s, (const u32*)en->vals.p,
-                       (const u64*)en->visited.p, (const int32_t*)g->inv, V, W, rshift,
-                       (const int32_t*)en->qmap.p, om, dout, vec);
-    HIP_CHECK(hipGetLastError());
-    en->tick(CAT_OUT, false);
+    if (!out_done) emit_output();
   }
   if (!wcc.empty()) {
     en->wcc_rows.ensure(wcc.size() * 4);
@@ -2915,6 +2956,7 @@ void engine_run_batch(cgp_engine* en, const cgp_query* queries, int32_t n, uint3
   HIP_CHECK(hipStreamSynchronize(en->s));
   st.wall_ms = now_ms() - t0;
   en->rep_wall_ms = st.wall_ms;
+  decode_rings();
   if (!out_dev) {
     HIP_CHECK(hipMemcpyAsync(out, dout, (size_t)n * V * 4, hipMemcpyDeviceToHost, en->s));
     HIP_CHECK(hipStreamSynchronize(en->s));
@@ -3518,6 +3560,7 @@ void cgp_engine_destroy(cgp_engine* en) {
   if (en->h_ctl) (void)hipHostFree(en->h_ctl);
   if (en->h_cfg) (void)hipHostFree(en->h_cfg);
   if (en->h_stage) (void)hipHostFree(en->h_stage);
+  if (en->h_rings) (void)hipHostFree(en->h_rings);
   if (en->ev_fork) (void)hipEventDestroy(en->ev_fork);
   if (en->ev_join) (void)hipEventDestroy(en->ev_join);
   if (en->s2) (void)hipStreamDestroy(en->s2);

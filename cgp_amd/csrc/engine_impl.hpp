@@ -153,10 +153,17 @@ struct cgp_engine {
   Cfg* h_cfg = nullptr;     // pinned
   int32_t* h_stage = nullptr;  // pinned staging (slot sources, qmap, wcc rows)
   size_t h_stage_cap = 0;
+  // Pinned landing area of the passes' iteration records, decoded into the
+  // report after the batch's final synchronisation.
+  cgpk::IterRec* h_rings = nullptr;
+  size_t h_rings_cap = 0;
   u32 stamp_epoch = 1;
   // Narrow words of the configuration the last launched filter ran with: its
   // narrow-word items wait for the next advance's narrow kernel.
   u32 narrow_binned = 0;
+  // Replays the engine's last pass took to finish: the next pass enqueues its
+  // output behind that replay, before reading the control block back.
+  int spec_replay = 0;
   bool next_dirty = false;
   bool busy = false;
   std::mutex mu;
@@ -287,6 +294,16 @@ struct cgp_engine {
     if (vals.ensure(elems * 4)) drop_graphs();
   }
 
+  // Capacity for `recs` iteration records; the caller has no copy into the
+  // old area in flight.
+  void stage_rings(size_t recs) {
+    if (recs <= h_rings_cap) return;
+    if (h_rings) (void)hipHostFree(h_rings);
+    h_rings = nullptr;
+    h_rings_cap = 0;
+    HIP_CHECK(hipHostMalloc((void**)&h_rings, recs * sizeof(cgpk::IterRec), hipHostMallocDefault));
+    h_rings_cap = recs;
+  }
   void stage(size_t ints) {
     if (ints <= h_stage_cap) return;
     if (h_stage) (void)hipHostFree(h_stage);

@@ -476,3 +476,30 @@ def test_run_report_per_iteration(gpu):
     assert all(rep[i]["next_items"] == rep[i + 1]["frontier_items"] for i in range(len(rep) - 1))
     assert abs(sum(r["advance_ms"] for r in rep) - st["advance_ms"]) < 1e-6 + 1e-6 * st["advance_ms"]
     assert all(r["dense"] in (0, 1) and r["pull"] in (0, 1) for r in rep)
+
+
+@pytest.mark.parametrize("out_dev", [False, True])
+def test_output_behind_the_expected_replay(gpu, out_dev):
+    # A batch enqueues its output behind the replay in which the engine's
+    # previous batch finished.  Sources without edges finish in the first
+    # replay; a weighted batch then needs several two-iteration replays, so
+    # its early output is overwritten once it finishes; the short batch after
+    # it finishes before the guessed replay, and its repeat where guessed.
+    h, g, og = rmat(12, 16, 41, 1, 50)
+    deg = np.diff(np.asarray(h.offsets))
+    iso = [int(x) for x in np.flatnonzero(deg == 0)[:3]]
+    src = [int(x) for x in h.sample_sources(40, seed=12)]
+    en = cgp_amd.Engine(g, graph_iterations=2)
+    short = ([BFS, SSSP, BFS], iso)
+    long_ = ([BFS] * 20 + [SSSP] * 20 + [WCC], src + [0])
+    for kinds, sources in [short, long_, short, short, long_, long_]:
+        say("output-behind-replay batch of", len(kinds), "out_dev", out_dev)
+        if out_dev:
+            buf = cgp_amd.DeviceBuffer(len(kinds) * g.V * 4)
+            _, st = en.run_batch(kinds, sources, out=buf)
+            out = buf.to_host(shape=(len(kinds), g.V))
+        else:
+            out, st = en.run_batch(kinds, sources)
+        assert np.array_equal(out, oracle_rows(og, kinds, sources))
+        if kinds is long_[0]:
+            assert st["iterations"] > 4  # more than two replays
