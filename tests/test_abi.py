@@ -123,3 +123,22 @@ def test_engine_option_defaults():
     assert o.dense_wide_addr == 0
     assert o.concurrent_advance == 0
     assert o.block_size == 256
+
+
+def test_renumber_choice_constants_and_validation():
+    # The header's CGP_RENUMBER_* values, their ctypes mirror, the Python
+    # argument mapping, and the rejection of an unknown choice, which comes
+    # before any device call.
+    text = open(HEADER).read()
+    for name, v in [("CGP_RENUMBER_AUTO", 0), ("CGP_RENUMBER_ON", 1), ("CGP_RENUMBER_OFF", 2)]:
+        assert re.search(r"#define %s %d\b" % (name, v), text), name
+    assert (_lib.RENUMBER_AUTO, _lib.RENUMBER_ON, _lib.RENUMBER_OFF) == (0, 1, 2)
+    assert [cgp_amd._renumber(r) for r in (None, True, False)] == [0, 1, 2]
+    off = (C.c_int32 * 3)(0, 1, 2)
+    idx = (C.c_int32 * 2)(1, 0)
+    out = C.c_void_p()
+    with pytest.raises(cgp_amd.CgpError) as e:
+        _lib.check(_lib.lib.cgp_graph_create_csr_ex(2, 2, off, idx, None, 7, C.byref(out)))
+    assert e.value.status == cgp_amd.CGP_EINVAL
+    assert "renumber" in e.value.message
+    assert not out.value
