@@ -104,7 +104,7 @@ def bytes_builder(w, W):
     return float(push), float(dense)
 
 
-def bytes_survey(w, W):
+def bytes_scope_formula(w, W):
     """The same launches priced by the per-frontier formula of the scope
     table: per frontier item 8 (offsets) + 8W (frontier words); per edge 4
     (index) + 4 (weight, when an SSSP bit is active) + 8 per active mask word
@@ -587,7 +587,7 @@ def main():
         same_counted &= bool(np.array_equal(out2.to_host(shape=(n, V)), batched_rows))
     del eng_c, out2
     push_b, dense_b = bytes_builder(work, W)
-    push_s, dense_s = bytes_survey(work, W)
+    push_s, dense_s = bytes_scope_formula(work, W)
     # The advance is two launches per iteration: k_adv_dense (lane-per-query
     # rows of dense SSSP words) and k_adv_push (everything else).  Each is
     # timed on the device clock in the timed steps; their algorithmic bytes
@@ -613,10 +613,10 @@ def main():
         b, bs, ms, nl = per_launch(k)
         kernels_report[k] = {"avg_launch_us": ms * 1e3, "launches_timed": nl, "bytes_per_launch": b,
                              "achieved_GBps": b / max(1e-12, ms / 1e3) / 1e9,
-                             "bytes_per_launch_survey_formula": bs,
-                             "achieved_GBps_survey_formula": bs / max(1e-12, ms / 1e3) / 1e9,
+                             "bytes_per_launch_scope_formula": bs,
+                             "achieved_GBps_scope_formula": bs / max(1e-12, ms / 1e3) / 1e9,
                              "ms_per_batch": kern[k][2] / args.steps}
-    bytes_per_launch, bytes_survey_launch, avg_launch_ms, k_launch = per_launch(dom)
+    bytes_per_launch, bytes_scope_launch, avg_launch_ms, k_launch = per_launch(dom)
     achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms > 0 else 0.0
     adv_total_ms = adv_ms / args.steps
     adv_total_b = (push_b + dense_b) / COUNTED_RUNS
@@ -658,10 +658,10 @@ def main():
             "launches_timed": k_launch,
             "byte_model": "engine: 4 B per (edge, active SSSP query) target value, 8 B per (edge, active BFS "
                           "word), per-item offsets and frontier words, 16 B per next word gaining bits",
-            "survey_formula": {
-                "bytes_per_launch": bytes_survey_launch,
-                "achieved": bytes_survey_launch / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms > 0 else 0.0,
-                "frac": bytes_survey_launch / (avg_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS if avg_launch_ms > 0
+            "scope_formula": {
+                "bytes_per_launch": bytes_scope_launch,
+                "achieved": bytes_scope_launch / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms > 0 else 0.0,
+                "frac": bytes_scope_launch / (avg_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS if avg_launch_ms > 0
                 else 0.0,
                 "model": "8 B per (edge, active mask word) whatever the number of active queries in it"},
             "advance_total": {"ms_per_batch": adv_total_ms, "bytes_per_batch": adv_total_b,

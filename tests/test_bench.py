@@ -72,6 +72,6 @@ def test_byte_models_from_work_counters():
     push, dense = bench.bytes_builder(w, 4)
     assert push == 10 * 40 + 100 * 4 + 40 * 4 + 70 * 8 + 90 * 4 + 7 * 4 + 5 * 16
     assert dense == 3 * 16 + 50 * 8 + 400 * 4 + 9 * 4 + 2 * 16
-    push, dense = bench.bytes_survey(w, 4)
+    push, dense = bench.bytes_scope_formula(w, 4)
     assert push == 10 * 40 + 100 * 4 + 40 * 4 + (70 + 30) * 8 + 5 * 16 + 7 * 4 + 12 * 4
     assert dense == 3 * 40 + 50 * 16 + 2 * 16 + 9 * 4 + 20 * 4
